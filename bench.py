@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X HemeLB timestep.
+
+Workload (one GPU): the seeded bifurcating vessel tree -- 5 generations (31
+capsule segments), root radius 40 voxels, Murray's law splits, 8^3 blocks --
+run with D3Q19 LBGK at tau = 0.7 in binary64, BFL walls with exact cut
+distances, a Ladd/Womersley root inlet with a 0.05 peak centreline velocity
+and 16 Nash zero-gauge pressure outlets.
+
+A step is one full LB timestep over every fluid site, inputs resident in HBM.
+Prints one JSON line: MLUPS, the dominant kernel's HBM roofline and the CPU
+oracle's rate on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0          # MI355X HBM3E
+METRIC = "MLUPS and % HBM roofline, D3Q19 vessel tree; wall time to periodic state"
+STORE_BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
+DTYPE = {"fp64": "f64", "fp32": "f32", "fp16": "f16"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def tree_config(storage: str):
+    import numpy as np
+
+    import hemelb_amd as H
+
+    t0 = time.time()
+    g = H.Geometry.vessel_tree(0, 5, 40.0)
+    dom = g.project(19)
+    log(f"[bench] geometry: {dom.n} fluid sites, {len(dom.outlets)} outlets, "
+        f"{time.time() - t0:.1f}s")
+    tau, period, peak = 0.7, 5000.0, 0.05
+    nu = (tau - 0.5) / 3.0
+    inl = dom.inlets[0]
+    omega = 2 * math.pi / period
+    alpha = inl.radius * math.sqrt(omega / nu)
+    z = complex(-math.sqrt(0.5), math.sqrt(0.5)) * alpha
+    s, term, k = 1 + 0j, 1 + 0j, 0
+    while abs(term) > 1e-17 * abs(s) or k < 5:
+        k += 1
+        term *= -(z * z) / (4 * k * k)
+        s += term
+    amp = abs(1 - 1 / s) / omega
+    inlets = [H.InOutLetWomersleyVelocity(tuple(inl.position), tuple(inl.normal),
+                                          radius=inl.radius,
+                                          pressure_gradient_amplitude=peak / amp,
+                                          period=period, womersley_number=alpha)]
+    outlets = [H.InOutLetCosine(tuple(o.position), tuple(o.normal), density_mean=1.0,
+                                density_amp=0.0, period=period) for o in dom.outlets]
+    traits = H.Traits(H.D3Q19, H.LBGK, H.BouzidiFirdaousLallemand, H.LaddIolet,
+                      H.NashZerothOrderPressure)
+    return dom, traits, tau, inlets, outlets, dict(period=period, womersley=alpha,
+                                                   pressure_gradient=peak / amp)
+
+
+def cpu_baseline(dom, tau, inlets, outlets, steps):
+    """The CPU oracle (binary64 restatement of the reference push scheme, one
+    core) on the same domain, timed over `steps` steps after one warm step."""
+    from oracle import bindings as O
+
+    ins = [O.make_iolet(O.ORC_WOMERSLEY, i.position, i.normal, radius=i.radius,
+                        pressure_gradient_amplitude=i.pressure_gradient_amplitude,
+                        period=i.period, womersley_number=i.womersley_number)
+           for i in inlets]
+    outs = [O.make_iolet(O.ORC_COSINE, o.position, o.normal, density_mean=o.density_mean,
+                         density_amp=o.density_amp, period=o.period) for o in outlets]
+    sim = O.OracleSim(19, O.ORC_LBGK, O.ORC_WALL_BFL, O.ORC_IOLET_LADD, O.ORC_IOLET_NASH,
+                      tau, dom.coords, dom.site_type, dom.wall_mask, dom.iolet_mask,
+                      dom.iolet_id, dom.cut, ins, outs)
+    sim.set_equilibrium(1.0)
+    sim.step(1)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    dt = time.perf_counter() - t0
+    return dom.n * steps / dt / 1e6, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--storage", choices=sorted(STORE_BYTES), default="fp64")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "path is not part of this build.", file=sys.stderr)
+        return 2
+
+    import hemelb_amd as H
+
+    dom, traits, tau, inlets, outlets, extra = tree_config(args.storage)
+    t0 = time.time()
+    lbm = H.LBM(traits, dom, H.LbmParameters(tau=tau), inlets, outlets,
+                storage=args.storage, rho0=1.0)
+    lbm.SetInitialConditions(1.0)
+    lbm.synchronize()
+    setup_s = time.time() - t0
+    log(f"[bench] solver set-up {setup_s:.1f}s, device memory "
+        f"{lbm.device_bytes() / 2**30:.2f} GiB")
+    lbm.step(args.warmup)
+    lbm.synchronize()
+    t_wall = time.perf_counter()
+    ms = lbm.time_steps(args.steps)
+    lbm.synchronize()
+    wall = time.perf_counter() - t_wall
+    mon = lbm.monitor()
+    if not mon["stable"]:
+        log("[bench] WARNING: state unstable after the timed steps")
+    n = dom.n
+    ms_per_step = ms / args.steps
+    mlups = n / (ms_per_step * 1e-3) / 1e6
+    b_alg = 2 * 19 * STORE_BYTES[args.storage]
+    achieved = b_alg * n / (ms_per_step * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}
+    prof = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(prof):
+        try:
+            rec = json.load(open(prof)).get(f"tree5_{args.storage}")
+            if rec and rec.get("sites") == n:
+                roof["traffic"] = rec["bytes_per_launch"]
+        except (OSError, ValueError):
+            pass
+    log(f"[bench] {args.steps} steps: {ms:.2f} ms on the solver stream "
+        f"({wall:.3f}s wall), {mlups:.1f} MLUPS, {achieved:.0f} GB/s algorithmic")
+    cpu = None
+    if not args.no_cpu_baseline:
+        c_mlups, c_s = cpu_baseline(dom, tau, inlets, outlets, args.cpu_steps)
+        cpu = {"value": round(c_mlups, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle (binary64 push scheme), same {n}-site tree, "
+                         f"{args.cpu_steps} steps after 1 warm step, {c_s:.1f}s"}
+        log(f"[bench] cpu oracle {c_mlups:.2f} MLUPS over {c_s:.1f}s")
+    out = {
+        "metric": METRIC, "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": DTYPE[args.storage], "data": "synthetic",
+        "config": {"workload": "vessel_tree_5gen_R40_D3Q19_LBGK_BFL_LaddWomersley_Nash",
+                   "fluid_sites": n, "lattice": "D3Q19", "kernel": "LBGK", "tau": tau,
+                   "walls": "BFL", "inlet": "Ladd/Womersley", "outlets": len(outlets),
+                   "storage": args.storage, "setup_s": round(setup_s, 2),
+                   "bytes_alg_per_site": b_alg, **{k: round(v, 6) for k, v in extra.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,94 @@
+// Internal interface between the host solver (solver.cpp, built with g++)
+// and the HIP kernels (lbm_kernels.hip, built with hipcc).  Plain structs of
+// device pointers; no HIP types so that the host side needs no device
+// compiler.
+#pragma once
+
+#include <cstdint>
+
+namespace hlb {
+
+constexpr int kMaxMoments = 15;
+constexpr int kMaxQ = 19;
+
+// Device-side description of one solver's domain in the library's own
+// layout: sites sorted into 8^3 blocks in Morton order, z-fastest inside a
+// block, distributions stored structure-of-arrays with stride `np`.
+struct DevDomain {
+  int64_t n = 0;    // fluid sites
+  int64_t np = 0;   // padded stride of every per-direction array
+  // Pull table: src[(d-1)*np + x] is the site whose post-collision value of
+  // direction d streams into slot d of site x (the site at x - c_d).  For a
+  // slot fed by a wall or iolet link it holds x itself (never dereferenced
+  // as a neighbour).
+  const uint32_t* src = nullptr;
+  // 0 for a site with no wall/iolet link, else its boundary record + 1.
+  const uint32_t* binfo = nullptr;
+  int64_t nb = 0;                  // boundary records
+  const uint32_t* b_wall = nullptr;   // [nb] wall link mask
+  const uint32_t* b_iolet = nullptr;  // [nb] iolet link mask
+  const int32_t* b_io = nullptr;      // [nb] iolet-site record or -1
+  const float* b_q = nullptr;         // [nb][Q-1] cut distances
+  int64_t ni = 0;                  // iolet-site records
+  const int32_t* io_iolet = nullptr;  // [ni] global iolet index (inlets first)
+  const int32_t* io_bc = nullptr;     // [ni] hlb_iolet_bc of the site's group
+  const int32_t* io_site = nullptr;   // [ni] internal site index
+  double* io_stash = nullptr;         // [ni] Nash: m.n/rho ; Ladd: rho
+  const double* io_ladd = nullptr;    // [ni][Q-1][2] per-link velocity coeffs
+  int32_t n_iolets = 0;
+  const float* iolet_nf = nullptr;    // [n_iolets][3] normal rounded to float
+  const double* iolet_n = nullptr;    // [n_iolets][3] normal
+  const int32_t* iolet_kind = nullptr;  // [n_iolets] hlb_iolet_kind
+};
+
+struct StepArgs {
+  int Q = 19;
+  int kernel = 0;     // hlb_kernel
+  int wall = 0;       // hlb_wall
+  int storage = 0;    // hlb_storage
+  double omega = 0;   // -1/tau
+  double rho0 = 1;    // shift for reduced-precision storage
+  float omega_f = 0;
+  // MRT: device arrays of (S_k * M[k][d] / diag_k), index k*Q + d
+  const double* mrt_c = nullptr;
+  const float* mrt_c_f = nullptr;
+  // Boundary values of the step being streamed: per iolet, two doubles
+  // (Nash: ghost density; Ladd Womersley: e^{i w t}; Ladd parabolic: speed).
+  const double* bv = nullptr;
+  int* unstable = nullptr;  // set to 1 if a post-stream f is not > 0
+};
+
+// Launches; every function returns 0 or a HIP error code.
+// Every site gets the post-stream distributions vals[Q] (binary64).
+int launch_fill(const StepArgs& a, int64_t n, int64_t np, const double* vals,
+                void* state, void* stream);
+int launch_step(const DevDomain& d, const StepArgs& a, const void* fin,
+                void* fout, void* stream);
+int launch_collide_only(const DevDomain& d, const StepArgs& a, const void* fin,
+                        void* fout, void* stream);
+// Post-stream distributions (the reference's fOld) as binary64 SoA [Q][np].
+// `from_post_stream` copies fin when the state already holds them.
+int launch_stream_only(const DevDomain& d, const StepArgs& a, const void* fin,
+                       double* s_out, int from_post_stream, void* stream);
+// SoA [Q][np] binary64 <-> caller-order AoS [n][Q] through perm
+// (perm[internal] = caller index).
+int launch_soa_to_aos(int Q, int64_t n, int64_t np, const double* soa,
+                      const int64_t* perm, double* aos, void* stream);
+int launch_aos_to_state(const StepArgs& a, int64_t n, int64_t np,
+                        const double* aos, const int64_t* perm, void* state,
+                        void* stream);
+// Moments of a binary64 SoA state: rho[np], u[3][np] (internal order).
+int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
+                   double* u, void* stream);
+// Fixed-order binary64 reductions over moments; partial results per block.
+int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
+                  const double* rho, const double* u, double* partials,
+                  int nblocks, void* stream);
+int launch_velocity_change(int64_t n, int64_t np, const double* u,
+                           const double* u_ref, double inv_ref, double* partials,
+                           int nblocks, void* stream);
+int launch_gather_pairs(int64_t count, int64_t np, const int64_t* slot_index,
+                        const void* state, int storage, double rho0,
+                        const double* w, double* out, void* stream);
+
+}  // namespace hlb

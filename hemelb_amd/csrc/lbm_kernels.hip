@@ -1,0 +1,810 @@
+// HIP kernels of the MI355X HemeLB timestep (gfx950).
+//
+// One thread updates one fluid site.  The state holds post-collision
+// distributions f*(t) structure-of-arrays (f[d*np + x]); a step pulls the
+// values streaming into every slot of a site (the reference pushes them,
+// Code/lb/streamers/BulkStreamer.h:31-39), resolves wall and iolet links in
+// registers with the reference's link priority (iolet first, then wall,
+// StreamerTypeFactory.h:65-78), applies the BFL PostStep correction
+// (BouzidiFirdaousLallemand.h:68-91) locally, collides (LBGK.h:140-174 or
+// MRT.h:56-105) and writes f*(t+1) to its own slots.  Binary64 arithmetic is
+// evaluated in the reference's SSE3 order without contraction, so the plain
+// method reproduces the reference bit for bit (up to the sign of zero).
+//
+// The bandwidth-bound path uses no MFMA.  Workgroups are remapped so that
+// each XCD sweeps one contiguous eighth of the Morton-ordered domain, which
+// keeps neighbour cache lines inside one L2.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "lbm_internal.h"
+
+namespace hlb {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Lattice constants (Code/lb/lattices/D3Q15.h:15-56, D3Q19.h:14-49)
+// ---------------------------------------------------------------------------
+template <int Q> struct Lat;
+template <> struct Lat<15> {
+  static constexpr int c[15][3] = {
+      {0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+      {0, 0, 1},   {0, 0, -1},  {1, 1, 1},  {-1, -1, -1}, {1, 1, -1},
+      {-1, -1, 1}, {1, -1, 1},  {-1, 1, -1}, {1, -1, -1}, {-1, 1, 1}};
+  static constexpr double w[15] = {
+      2.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,
+      1.0 / 9.0,  1.0 / 9.0,  1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0,
+      1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0};
+  static constexpr int nmom = 11;
+  // DHumieresD3Q15MRTBasis::REDUCED_MOMENT_BASIS (DHumieresD3Q15MRTBasis.h:41-53)
+  static constexpr int M[11][15] = {
+      {-2, -1, -1, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {16, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1},
+      {0, -4, 4, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1},
+      {0, 0, 0, -4, 4, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1},
+      {0, 0, 0, 0, 0, -4, 4, 1, -1, -1, 1, 1, -1, -1, 1},
+      {0, 2, 2, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, -1, -1, 1, 1},
+      {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1}};
+};
+template <> struct Lat<19> {
+  static constexpr int c[19][3] = {
+      {0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+      {0, 0, 1},  {0, 0, -1},  {1, 1, 0},  {-1, -1, 0}, {1, -1, 0},
+      {-1, 1, 0}, {1, 0, 1},   {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1},
+      {0, 1, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}};
+  static constexpr double w[19] = {
+      1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+      1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+  static constexpr int nmom = 15;
+  // DHumieresD3Q19MRTBasis::REDUCED_MOMENT_BASIS (DHumieresD3Q19MRTBasis.h:38-53)
+  static constexpr int M[15][19] = {
+      {-30, -11, -11, -11, -11, -11, -11, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
+      {12, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {0, -4, 4, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, -4, 4, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, -4, 4, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1},
+      {0, 2, 2, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, -4, -4, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, 0, 0, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, -2, -2, 2, 2, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1}};
+};
+
+// Inverse directions: both lattices pair d with d+1 for odd d
+// (detail::compute_inverses, Lattice.h:49-69).
+__host__ __device__ constexpr int inv_dir(int d) {
+  return d == 0 ? 0 : ((d & 1) ? d + 1 : d - 1);
+}
+
+constexpr double kCs2 = 1.0 / 3.0;  // constants.h:41
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    sfor<I + 1, N>(f);
+  }
+}
+
+// acc + C*v for a lattice component C in {-1,0,1}; the zero term is dropped
+// (it can only change the sign of an exact zero).
+template <int C, typename T>
+__device__ __forceinline__ T addc(T acc, T v) {
+  if constexpr (C == 0) return acc;
+  else if constexpr (C > 0) return acc + v;
+  else return acc - v;
+}
+
+// (C0*a + C1*b) + C2*c as the reference evaluates the projection of the
+// momentum on a lattice vector (Lattice.h:276-285).
+template <int C0, int C1, int C2, typename T>
+__device__ __forceinline__ T dotc(T a, T b, T c) {
+  constexpr bool h0 = C0 != 0, h1 = C1 != 0, h2 = C2 != 0;
+  const T s0 = C0 > 0 ? a : -a;
+  const T s1 = C1 > 0 ? b : -b;
+  const T s2 = C2 > 0 ? c : -c;
+  if constexpr (h0 && h1 && h2) return (s0 + s1) + s2;
+  else if constexpr (h0 && h1) return s0 + s1;
+  else if constexpr (h0 && h2) return s0 + s2;
+  else if constexpr (h1 && h2) return s1 + s2;
+  else if constexpr (h0) return s0;
+  else if constexpr (h1) return s1;
+  else if constexpr (h2) return s2;
+  else return T(0);
+}
+
+// ---------------------------------------------------------------------------
+// Storage: binary64 holds f; binary32/binary16 hold f - w_i*rho0.
+// ---------------------------------------------------------------------------
+template <typename S> struct Store;
+template <> struct Store<double> {
+  using T = double;
+  static __device__ __forceinline__ double ld(const double* p) { return *p; }
+  static __device__ __forceinline__ void st(double* p, double v) { *p = v; }
+};
+template <> struct Store<float> {
+  using T = float;
+  static __device__ __forceinline__ float ld(const float* p) { return *p; }
+  static __device__ __forceinline__ void st(float* p, float v) { *p = v; }
+};
+template <> struct Store<_Float16> {
+  using T = float;
+  static __device__ __forceinline__ float ld(const _Float16* p) { return (float)*p; }
+  static __device__ __forceinline__ void st(_Float16* p, float v) { *p = (_Float16)v; }
+};
+
+template <typename S> constexpr bool kShifted = !std::is_same<S, double>::value;
+
+// ---------------------------------------------------------------------------
+// Moments and equilibrium
+// ---------------------------------------------------------------------------
+// Lattice::CalculateDensityAndMomentum, SSE3 order (Lattice.h:119-178):
+// lane 0 from 0 over even directions, lane 1 from the last (odd Q) direction
+// over odd directions; result lane0 + lane1.  For shifted storage the same
+// sums give rho - rho0 and the momentum.
+template <int Q, typename T>
+__device__ __forceinline__ void moments(const T (&f)[Q], T& rho, T& mx, T& my,
+                                        T& mz) {
+  static_assert(Q & 1, "odd lattices only");
+  using L = Lat<Q>;
+  T r0 = T(0), r1 = f[Q - 1];
+  T x0 = T(0), y0 = T(0), z0 = T(0);
+  T x1 = addc<L::c[Q - 1][0]>(T(0), f[Q - 1]);
+  T y1 = addc<L::c[Q - 1][1]>(T(0), f[Q - 1]);
+  T z1 = addc<L::c[Q - 1][2]>(T(0), f[Q - 1]);
+  sfor<0, (Q - 1) / 2>([&](auto p) {
+    constexpr int d = 2 * decltype(p)::value;
+    r0 = r0 + f[d];
+    r1 = r1 + f[d + 1];
+    x0 = addc<L::c[d][0]>(x0, f[d]);
+    x1 = addc<L::c[d + 1][0]>(x1, f[d + 1]);
+    y0 = addc<L::c[d][1]>(y0, f[d]);
+    y1 = addc<L::c[d + 1][1]>(y1, f[d + 1]);
+    z0 = addc<L::c[d][2]>(z0, f[d]);
+    z1 = addc<L::c[d + 1][2]>(z1, f[d + 1]);
+  });
+  rho = r0 + r1;
+  mx = x0 + x1;
+  my = y0 + y1;
+  mz = z0 + z1;
+}
+
+// Common factors of Lattice::CalculateFeq, SSE3 variant (Lattice.h:236-270).
+template <typename T>
+struct FeqCoef {
+  T t1, nine, mx, my, mz;
+  __device__ __forceinline__ FeqCoef(T rho, T mx_, T my_, T mz_)
+      : mx(mx_), my(my_), mz(mz_) {
+    const T three_halves = T(3. / 2.) * ((mx * mx + my * my) + mz * mz);
+    t1 = rho - three_halves / rho;
+    const T rho_1 = T(1.) / rho;
+    nine = T(9. / 2.);
+    nine *= rho_1;
+  }
+  // f_eq[i] = w_i * ((t1 + nine*(e*e)) + 3*e)  (Lattice.h:271-320)
+  template <int Q, int I>
+  __device__ __forceinline__ T eq() const {
+    using L = Lat<Q>;
+    const T e = dotc<L::c[I][0], L::c[I][1], L::c[I][2]>(mx, my, mz);
+    return T(L::w[I]) * ((t1 + nine * (e * e)) + T(3.) * e);
+  }
+};
+
+// Shifted equilibrium feq_i - w_i*rho0 for reduced-precision storage, with
+// drho = rho - rho0 carried separately (no cancellation against rho0).
+template <typename T>
+struct FeqShifted {
+  T drho, q, nine, mx, my, mz;
+  __device__ __forceinline__ FeqShifted(T drho_, T rho, T mx_, T my_, T mz_)
+      : drho(drho_), mx(mx_), my(my_), mz(mz_) {
+    const T inv = T(1) / rho;
+    q = drho - T(1.5) * ((mx * mx + my * my) + mz * mz) * inv;
+    nine = T(4.5) * inv;
+  }
+  template <int Q, int I>
+  __device__ __forceinline__ T eq() const {
+    using L = Lat<Q>;
+    const T e = dotc<L::c[I][0], L::c[I][1], L::c[I][2]>(mx, my, mz);
+    return T(L::w[I]) * ((q + nine * (e * e)) + T(3.) * e);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Collision
+// ---------------------------------------------------------------------------
+template <int Q, typename T>
+struct Hydro {
+  T rho, mx, my, mz;  // rho is the full density (rho0 added back if shifted)
+};
+
+// Collide in place: f (post-stream, possibly shifted) -> f* ; returns the
+// moments the streamers of the next step need.
+template <int Q, int KERNEL, bool SHIFT, typename T>
+__device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
+  using L = Lat<Q>;
+  Hydro<Q, T> h;
+  T r;
+  moments<Q, T>(f, r, h.mx, h.my, h.mz);
+  T fneq[Q];
+  if constexpr (SHIFT) {
+    const T rho = T(a.rho0) + r;
+    h.rho = rho;
+    FeqShifted<T> fe(r, rho, h.mx, h.my, h.mz);
+    sfor<0, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      fneq[d] = f[d] - fe.template eq<Q, d>();
+    });
+  } else {
+    h.rho = r;
+    FeqCoef<T> fe(r, h.mx, h.my, h.mz);
+    sfor<0, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      fneq[d] = f[d] - fe.template eq<Q, d>();
+    });
+  }
+  if constexpr (KERNEL == 0) {
+    // LBGK::Collide: f + f_neq * omega (LBGK.h:167-174)
+    const T omega = std::is_same<T, double>::value ? T(a.omega) : T(a.omega_f);
+    sfor<0, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      f[d] = f[d] + fneq[d] * omega;
+    });
+  } else {
+    // MRT: m_neq = M f_neq (MRT.h:123-135); f* = f - sum_k (S_k N_kd) m_neq_k
+    constexpr int NM = L::nmom;
+    T mneq[NM];
+    sfor<0, NM>([&](auto ki) {
+      constexpr int k = decltype(ki)::value;
+      T acc = T(0.);
+      sfor<0, Q>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        constexpr int m = L::M[k][i];
+        if constexpr (m == 1) acc = acc + fneq[i];
+        else if constexpr (m == -1) acc = acc - fneq[i];
+        else if constexpr (m != 0) acc = acc + T(m) * fneq[i];
+      });
+      mneq[k] = acc;
+    });
+    const T* __restrict__ C;
+    if constexpr (std::is_same<T, double>::value) C = a.mrt_c; else C = a.mrt_c_f;
+    sfor<0, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      T coll = T(0.);
+      sfor<0, NM>([&](auto ki) {
+        constexpr int k = decltype(ki)::value;
+        if constexpr (L::M[k][d] != 0) coll += C[k * Q + d] * mneq[k];
+      });
+      f[d] = f[d] - coll;
+    });
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// Streaming (pull) with boundary links
+// ---------------------------------------------------------------------------
+// Ghost equilibrium of the Nash zeroth-order pressure link
+// (NashZerothOrderPressure.h:27-60): density of the iolet, momentum =
+// (n_float * (m.n_float / rho)) * rho_ghost, one component of
+// Lattice::CalculateFeq.  Always binary64.
+template <int Q, int E>
+__device__ __forceinline__ double nash_value(const DevDomain& D, const StepArgs& A,
+                                             int io, int k) {
+  const double rho_g = A.bv[2 * k];
+  const double comp = D.io_stash[io];
+  const double gx = (double)D.iolet_nf[3 * k + 0] * comp * rho_g;
+  const double gy = (double)D.iolet_nf[3 * k + 1] * comp * rho_g;
+  const double gz = (double)D.iolet_nf[3 * k + 2] * comp * rho_g;
+  FeqCoef<double> fe(rho_g, gx, gy, gz);
+  return fe.template eq<Q, E>();
+}
+
+// Correction of the Ladd link (LaddIolet.h:29-66): 2 w_d (rho u_w . c_d)/Cs2
+// with u_w the iolet velocity half way along the link.
+template <int Q, int DP>
+__device__ __forceinline__ double ladd_correction(const DevDomain& D,
+                                                  const StepArgs& A, int io,
+                                                  int k) {
+  using L = Lat<Q>;
+  const double rho = D.io_stash[io];
+  const double* lc = D.io_ladd + ((int64_t)io * (Q - 1) + DP - 1) * 2;
+  const double nx = D.iolet_n[3 * k], ny = D.iolet_n[3 * k + 1],
+               nz = D.iolet_n[3 * k + 2];
+  double ux, uy, uz;
+  if (D.iolet_kind[k] == 2) {
+    // InOutLetWomersleyVelocity::GetVelocity: Re[A e^{i w t}], u = n * -vm
+    const double vm = lc[0] * A.bv[2 * k] - lc[1] * A.bv[2 * k + 1];
+    ux = nx * -vm;
+    uy = ny * -vm;
+    uz = nz * -vm;
+  } else {
+    // InOutLetParabolicVelocity::GetVelocity: n * (max * (1 - r^2/R^2))
+    const double s = A.bv[2 * k] * lc[0];
+    ux = nx * s;
+    uy = ny * s;
+    uz = nz * s;
+  }
+  ux *= rho;
+  uy *= rho;
+  uz *= rho;
+  const double dot = dotc<L::c[DP][0], L::c[DP][1], L::c[DP][2]>(ux, uy, uz);
+  return 2. * L::w[DP] * dot / kCs2;
+}
+
+// Post-stream distributions of site x (the reference's fNew after
+// StreamAndCollide and PostStep) from the post-collision state `fin`.
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
+                                       const S* __restrict__ fin, int64_t x,
+                                       T (&f)[Q]) {
+  using St = Store<S>;
+  const int64_t np = D.np;
+  const uint32_t bi = D.binfo[x];
+  f[0] = St::ld(fin + x);
+  if (bi == 0) {
+    sfor<1, Q>([&](auto ei) {
+      constexpr int e = decltype(ei)::value;
+      const uint32_t s = D.src[(int64_t)(e - 1) * np + x];
+      f[e] = St::ld(fin + (int64_t)e * np + s);
+    });
+    return;
+  }
+  const uint32_t rec = bi - 1;
+  const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
+  const int io = D.b_io[rec];
+  const int k = io >= 0 ? D.io_iolet[io] : 0;
+  const int bc = io >= 0 ? D.io_bc[io] : 0;
+  const float* q = D.b_q + (int64_t)rec * (Q - 1);
+  sfor<1, Q>([&](auto ei) {
+    constexpr int e = decltype(ei)::value;
+    constexpr int dp = inv_dir(e);  // link of x that points at the source
+    constexpr uint32_t bit = 1u << (dp - 1);
+    if (im & bit) {
+      if (bc == 0) {
+        double v = nash_value<Q, e>(D, A, io, k);
+        if constexpr (kShifted<S>) v -= Lat<Q>::w[e] * A.rho0;
+        f[e] = (T)v;
+      } else {
+        f[e] = St::ld(fin + (int64_t)dp * np + x) -
+               (T)ladd_correction<Q, dp>(D, A, io, k);
+      }
+    } else if (wm & bit) {
+      const T own = St::ld(fin + (int64_t)dp * np + x);
+      if (A.wall == 0) {
+        f[e] = own;  // BounceBackLink::StreamLink
+      } else {
+        const T qd = (T)(double)q[dp - 1];
+        if ((wm & (1u << (e - 1))) || qd < T(0.5)) {
+          f[e] = own;
+        } else {
+          const T opp = St::ld(fin + (int64_t)e * np + x);
+          f[e] = (own + (T(2.0) * qd - T(1)) * opp) / (T(2.0) * qd);
+        }
+      }
+    } else {
+      const uint32_t s = D.src[(int64_t)(e - 1) * np + x];
+      f[e] = St::ld(fin + (int64_t)e * np + s);
+    }
+  });
+  if (A.wall == 1 && wm) {
+    // BouzidiFirdaousLallemandLink::PostStepLink
+    sfor<1, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      constexpr int id = inv_dir(d);
+      if (wm & (1u << (d - 1))) {
+        const T qd = (T)(double)q[d - 1];
+        if (!(wm & (1u << (id - 1))) && qd < T(0.5))
+          f[id] = T(2.0) * qd * f[id] + (T(1.0) - T(2.0) * qd) * f[d];
+      }
+    });
+  }
+}
+
+__device__ __forceinline__ int64_t site_index() {
+  // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give
+  // every XCD one contiguous range of logical workgroups.
+  const uint32_t nwg = gridDim.x, b = blockIdx.x;
+  const uint32_t xcd = b & 7u, k = b >> 3;
+  const uint32_t base = nwg >> 3, rem = nwg & 7u;
+  const uint32_t start = xcd * base + (xcd < rem ? xcd : rem);
+  const uint32_t logical = start + k;
+  return (int64_t)logical * blockDim.x + threadIdx.x;
+}
+
+template <int Q, typename T>
+__device__ __forceinline__ void update_stash(const DevDomain& D, int64_t x,
+                                             const Hydro<Q, T>& h) {
+  const uint32_t bi = D.binfo[x];
+  if (bi == 0) return;
+  const int io = D.b_io[bi - 1];
+  if (io < 0) return;
+  const double rho = (double)h.rho;
+  if (D.io_bc[io] == 0) {
+    // component = Dot(momentum, n_float) / density (NashZerothOrderPressure.h:43)
+    const int k = D.io_iolet[io];
+    double acc = 0.0;
+    acc = acc + (double)h.mx * (double)D.iolet_nf[3 * k];
+    acc = acc + (double)h.my * (double)D.iolet_nf[3 * k + 1];
+    acc = acc + (double)h.mz * (double)D.iolet_nf[3 * k + 2];
+    D.io_stash[io] = acc / rho;
+  } else {
+    D.io_stash[io] = rho;  // LaddIolet.h:55-58 (compressible lattice)
+  }
+}
+
+template <int Q, int KERNEL, typename S>
+__global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
+                                              const S* __restrict__ fin,
+                                              S* __restrict__ fout) {
+  using T = typename Store<S>::T;
+  const int64_t x = site_index();
+  if (x >= D.n) return;
+  T f[Q];
+  gather<Q, S, T>(D, A, fin, x, f);
+  bool ok = true;
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    if constexpr (kShifted<S>)
+      ok = ok && ((double)f[d] + Lat<Q>::w[d] * A.rho0 > 0.0);
+    else
+      ok = ok && (f[d] > T(0.0));
+  });
+  if (!ok) atomicOr(A.unstable, 1);
+  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, A);
+  update_stash<Q, T>(D, x, h);
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    Store<S>::st(fout + (int64_t)d * D.np + x, f[d]);
+  });
+}
+
+template <int Q, int KERNEL, typename S>
+__global__ __launch_bounds__(256) void k_collide_only(DevDomain D, StepArgs A,
+                                                      const S* __restrict__ fin,
+                                                      S* __restrict__ fout) {
+  using T = typename Store<S>::T;
+  const int64_t x = site_index();
+  if (x >= D.n) return;
+  T f[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    f[d] = Store<S>::ld(fin + (int64_t)d * D.np + x);
+  });
+  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, A);
+  update_stash<Q, T>(D, x, h);
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    Store<S>::st(fout + (int64_t)d * D.np + x, f[d]);
+  });
+}
+
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_stream_only(DevDomain D, StepArgs A,
+                                                     const S* __restrict__ fin,
+                                                     double* __restrict__ out,
+                                                     int from_post_stream) {
+  using T = typename Store<S>::T;
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= D.n) return;
+  T f[Q];
+  if (from_post_stream) {
+    sfor<0, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      f[d] = Store<S>::ld(fin + (int64_t)d * D.np + x);
+    });
+  } else {
+    gather<Q, S, T>(D, A, fin, x, f);
+  }
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    double v = (double)f[d];
+    if constexpr (kShifted<S>) v += Lat<Q>::w[d] * A.rho0;
+    out[(int64_t)d * D.np + x] = v;
+  });
+}
+
+__global__ void k_soa_to_aos(int Q, int64_t n, int64_t np, const double* soa,
+                             const int64_t* perm, double* aos) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const int64_t c = perm[x];
+  for (int d = 0; d < Q; ++d) aos[c * Q + d] = soa[(int64_t)d * np + x];
+}
+
+__global__ void k_aos_to_state(int Q, int64_t n, int64_t np, const double* aos,
+                               const int64_t* perm, double* state) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const int64_t c = perm[x];
+  for (int d = 0; d < Q; ++d) state[(int64_t)d * np + x] = aos[c * Q + d];
+}
+
+template <int Q>
+__global__ void k_aos_to_shifted(int64_t n, int64_t np, const double* aos,
+                                 const int64_t* perm, float* s32, _Float16* s16,
+                                 double rho0) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const int64_t c = perm[x];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    const double v = aos[c * Q + d] - Lat<Q>::w[d] * rho0;
+    if (s32) s32[(int64_t)d * np + x] = (float)v;
+    else s16[(int64_t)d * np + x] = (_Float16)(float)v;
+  });
+}
+
+template <int Q>
+__global__ void k_moments(int64_t n, int64_t np, const double* soa, double* rho,
+                          double* u) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  double f[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    f[d] = soa[(int64_t)d * np + x];
+  });
+  double r, mx, my, mz;
+  moments<Q, double>(f, r, mx, my, mz);
+  rho[x] = r;
+  u[x] = mx / r;
+  u[np + x] = my / r;
+  u[2 * np + x] = mz / r;
+}
+
+// Per-block binary64 reductions in a fixed tree order: [mass, min rho,
+// max rho, max |u|, unstable] per block.
+template <int Q>
+__global__ __launch_bounds__(256) void k_reduce(int64_t n, int64_t np,
+                                                const double* soa,
+                                                const double* rho,
+                                                const double* u,
+                                                double* partials) {
+  __shared__ double s_sum[256], s_min[256], s_max[256], s_u[256], s_bad[256];
+  const int t = threadIdx.x;
+  double sum = 0.0, mn = 1.7976931348623157e308, mx = -1.7976931348623157e308,
+         um = 0.0, bad = 0.0;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + t; x < n;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const double r = rho[x];
+    sum += r;
+    mn = r < mn ? r : mn;
+    mx = r > mx ? r : mx;
+    const double a = u[x], b = u[np + x], c = u[2 * np + x];
+    const double m = sqrt(((0.0 + a * a) + b * b) + c * c);
+    um = m > um ? m : um;
+    for (int d = 0; d < Q; ++d)
+      if (!(soa[(int64_t)d * np + x] > 0.0)) bad = 1.0;
+  }
+  s_sum[t] = sum; s_min[t] = mn; s_max[t] = mx; s_u[t] = um; s_bad[t] = bad;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s_sum[t] += s_sum[t + w];
+      s_min[t] = s_min[t + w] < s_min[t] ? s_min[t + w] : s_min[t];
+      s_max[t] = s_max[t + w] > s_max[t] ? s_max[t + w] : s_max[t];
+      s_u[t] = s_u[t + w] > s_u[t] ? s_u[t + w] : s_u[t];
+      s_bad[t] = s_bad[t + w] > s_bad[t] ? s_bad[t + w] : s_bad[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* p = partials + 5 * blockIdx.x;
+    p[0] = s_sum[0]; p[1] = s_min[0]; p[2] = s_max[0]; p[3] = s_u[0]; p[4] = s_bad[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_velocity_change(int64_t n, int64_t np,
+                                                         const double* u,
+                                                         const double* u0,
+                                                         double inv_ref,
+                                                         double* partials) {
+  __shared__ double s[256];
+  const int t = threadIdx.x;
+  double m = 0.0;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + t; x < n;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const double a = u[x] - u0[x], b = u[np + x] - u0[np + x],
+                 c = u[2 * np + x] - u0[2 * np + x];
+    const double v = sqrt(((0.0 + a * a) + b * b) + c * c) * inv_ref;
+    m = v > m ? v : m;
+  }
+  s[t] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) s[t] = s[t + w] > s[t] ? s[t + w] : s[t];
+    __syncthreads();
+  }
+  if (t == 0) partials[blockIdx.x] = s[0];
+}
+
+template <typename S>
+__global__ void k_gather_pairs(int64_t count, int64_t np, const int64_t* slot,
+                               const S* state, double rho0, const double* w,
+                               double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t sidx = slot[i];
+  double v = (double)Store<S>::ld(state + sidx);
+  if constexpr (kShifted<S>) v += w[sidx / np] * rho0;
+  out[i] = v;
+}
+
+template <int Q, typename S>
+__global__ void k_fill(int64_t n, int64_t np, const double* vals, S* state,
+                       double rho0) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    double v = vals[d];
+    if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
+    Store<S>::st(state + (int64_t)d * np + x, (typename Store<S>::T)v);
+  });
+}
+
+inline dim3 grid_for(int64_t n, int threads = 256) {
+  return dim3((unsigned)((n + threads - 1) / threads));
+}
+
+template <int Q, int KERNEL, typename S>
+int step_t(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
+           hipStream_t st) {
+  hipLaunchKernelGGL((k_step<Q, KERNEL, S>), grid_for(d.n), dim3(256), 0, st, d,
+                     a, (const S*)fin, (S*)fout);
+  return (int)hipGetLastError();
+}
+
+template <int Q, int KERNEL, typename S>
+int collide_t(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
+              hipStream_t st) {
+  hipLaunchKernelGGL((k_collide_only<Q, KERNEL, S>), grid_for(d.n), dim3(256), 0,
+                     st, d, a, (const S*)fin, (S*)fout);
+  return (int)hipGetLastError();
+}
+
+template <typename Fn>
+int dispatch(const StepArgs& a, Fn&& fn) {
+  // (lattice, kernel, storage) -> instantiation
+#define HLB_CASE(Q_, K_, S_) \
+  if (a.Q == Q_ && a.kernel == K_ && a.storage == (std::is_same<S_, double>::value ? 0 : std::is_same<S_, float>::value ? 1 : 2)) \
+    return fn(std::integral_constant<int, Q_>{}, std::integral_constant<int, K_>{}, (S_*)nullptr);
+  HLB_CASE(15, 0, double) HLB_CASE(15, 1, double) HLB_CASE(19, 0, double) HLB_CASE(19, 1, double)
+  HLB_CASE(15, 0, float) HLB_CASE(15, 1, float) HLB_CASE(19, 0, float) HLB_CASE(19, 1, float)
+  HLB_CASE(15, 0, _Float16) HLB_CASE(15, 1, _Float16) HLB_CASE(19, 0, _Float16) HLB_CASE(19, 1, _Float16)
+#undef HLB_CASE
+  return (int)hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int launch_fill(const StepArgs& a, int64_t n, int64_t np, const double* vals,
+                void* state, void* stream) {
+  return dispatch(a, [&](auto q, auto, auto* s) {
+    using S = std::remove_pointer_t<decltype(s)>;
+    constexpr int Q = decltype(q)::value;
+    hipLaunchKernelGGL((k_fill<Q, S>), grid_for(n), dim3(256), 0,
+                       (hipStream_t)stream, n, np, vals, (S*)state, a.rho0);
+    return (int)hipGetLastError();
+  });
+}
+
+int launch_step(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
+                void* stream) {
+  return dispatch(a, [&](auto q, auto k, auto* s) {
+    using S = std::remove_pointer_t<decltype(s)>;
+    return step_t<decltype(q)::value, decltype(k)::value, S>(d, a, fin, fout,
+                                                             (hipStream_t)stream);
+  });
+}
+
+int launch_collide_only(const DevDomain& d, const StepArgs& a, const void* fin,
+                        void* fout, void* stream) {
+  return dispatch(a, [&](auto q, auto k, auto* s) {
+    using S = std::remove_pointer_t<decltype(s)>;
+    return collide_t<decltype(q)::value, decltype(k)::value, S>(
+        d, a, fin, fout, (hipStream_t)stream);
+  });
+}
+
+int launch_stream_only(const DevDomain& d, const StepArgs& a, const void* fin,
+                       double* s_out, int from_post_stream, void* stream) {
+  return dispatch(a, [&](auto q, auto, auto* s) {
+    using S = std::remove_pointer_t<decltype(s)>;
+    constexpr int Q = decltype(q)::value;
+    hipLaunchKernelGGL((k_stream_only<Q, S>), grid_for(d.n), dim3(256), 0,
+                       (hipStream_t)stream, d, a, (const S*)fin, s_out,
+                       from_post_stream);
+    return (int)hipGetLastError();
+  });
+}
+
+int launch_soa_to_aos(int Q, int64_t n, int64_t np, const double* soa,
+                      const int64_t* perm, double* aos, void* stream) {
+  hipLaunchKernelGGL(k_soa_to_aos, grid_for(n), dim3(256), 0, (hipStream_t)stream,
+                     Q, n, np, soa, perm, aos);
+  return (int)hipGetLastError();
+}
+
+int launch_aos_to_state(const StepArgs& a, int64_t n, int64_t np,
+                        const double* aos, const int64_t* perm, void* state,
+                        void* stream) {
+  if (a.storage == 0) {
+    hipLaunchKernelGGL(k_aos_to_state, grid_for(n), dim3(256), 0,
+                       (hipStream_t)stream, a.Q, n, np, aos, perm,
+                       (double*)state);
+    return (int)hipGetLastError();
+  }
+  float* s32 = a.storage == 1 ? (float*)state : nullptr;
+  _Float16* s16 = a.storage == 2 ? (_Float16*)state : nullptr;
+  if (a.Q == 15)
+    hipLaunchKernelGGL(k_aos_to_shifted<15>, grid_for(n), dim3(256), 0,
+                       (hipStream_t)stream, n, np, aos, perm, s32, s16, a.rho0);
+  else
+    hipLaunchKernelGGL(k_aos_to_shifted<19>, grid_for(n), dim3(256), 0,
+                       (hipStream_t)stream, n, np, aos, perm, s32, s16, a.rho0);
+  return (int)hipGetLastError();
+}
+
+int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
+                   double* u, void* stream) {
+  if (Q == 15)
+    hipLaunchKernelGGL(k_moments<15>, grid_for(n), dim3(256), 0,
+                       (hipStream_t)stream, n, np, soa, rho, u);
+  else
+    hipLaunchKernelGGL(k_moments<19>, grid_for(n), dim3(256), 0,
+                       (hipStream_t)stream, n, np, soa, rho, u);
+  return (int)hipGetLastError();
+}
+
+int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
+                  const double* rho, const double* u, double* partials,
+                  int nblocks, void* stream) {
+  if (Q == 15)
+    hipLaunchKernelGGL(k_reduce<15>, dim3(nblocks), dim3(256), 0,
+                       (hipStream_t)stream, n, np, soa, rho, u, partials);
+  else
+    hipLaunchKernelGGL(k_reduce<19>, dim3(nblocks), dim3(256), 0,
+                       (hipStream_t)stream, n, np, soa, rho, u, partials);
+  return (int)hipGetLastError();
+}
+
+int launch_velocity_change(int64_t n, int64_t np, const double* u,
+                           const double* u_ref, double inv_ref, double* partials,
+                           int nblocks, void* stream) {
+  hipLaunchKernelGGL(k_velocity_change, dim3(nblocks), dim3(256), 0,
+                     (hipStream_t)stream, n, np, u, u_ref, inv_ref, partials);
+  return (int)hipGetLastError();
+}
+
+int launch_gather_pairs(int64_t count, int64_t np, const int64_t* slot_index,
+                        const void* state, int storage, double rho0,
+                        const double* w, double* out, void* stream) {
+  if (storage == 0)
+    hipLaunchKernelGGL(k_gather_pairs<double>, grid_for(count), dim3(256), 0,
+                       (hipStream_t)stream, count, np, slot_index,
+                       (const double*)state, rho0, w, out);
+  else if (storage == 1)
+    hipLaunchKernelGGL(k_gather_pairs<float>, grid_for(count), dim3(256), 0,
+                       (hipStream_t)stream, count, np, slot_index,
+                       (const float*)state, rho0, w, out);
+  else
+    hipLaunchKernelGGL(k_gather_pairs<_Float16>, grid_for(count), dim3(256), 0,
+                       (hipStream_t)stream, count, np, slot_index,
+                       (const _Float16*)state, rho0, w, out);
+  return (int)hipGetLastError();
+}
+
+}  // namespace hlb

@@ -1,0 +1,967 @@
+// Host side of the MI355X HemeLB timestep behind the C ABI in hemelb_amd.h.
+// Built with g++ (-ffp-contract=off) so that the boundary values computed on
+// the host (cosine densities, Womersley profiles via the complex Bessel
+// series) use the same compiler runtime and libm as the reference's g++
+// build and its CPU oracle.
+#include "hemelb_amd.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "lbm_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+constexpr double kPi = 3.14159265358979323846264338327950288;  // constants.h:18
+constexpr double kCs2 = 1.0 / 3.0;                             // constants.h:41
+constexpr double kMmHgToPa = 133.3223874;                      // constants.h:20
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OR_FAIL(expr)                                                    \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(HLB_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Lattice tables (Code/lb/lattices/D3Q15.h:15-56, D3Q19.h:14-49).
+const int kC15[15][3] = {{0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},
+                         {0, -1, 0},  {0, 0, 1},   {0, 0, -1}, {1, 1, 1},
+                         {-1, -1, -1}, {1, 1, -1}, {-1, -1, 1}, {1, -1, 1},
+                         {-1, 1, -1}, {1, -1, -1}, {-1, 1, 1}};
+const double kW15[15] = {2.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,
+                         1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 72.0,
+                         1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0,
+                         1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0};
+const int kC19[19][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},
+                         {0, -1, 0}, {0, 0, 1},  {0, 0, -1}, {1, 1, 0},
+                         {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 0, 1},
+                         {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {0, 1, 1},
+                         {0, -1, -1}, {0, 1, -1}, {0, -1, 1}};
+const double kW19[19] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                         1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0,
+                         1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                         1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                         1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+// d'Humieres reduced moment bases (DHumieresD3Q15MRTBasis.h:41-53,
+// DHumieresD3Q19MRTBasis.h:38-53) and relaxation rates (*.cc SetUpCollisionMatrix).
+const int kM15[11][15] = {
+    {-2, -1, -1, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1},
+    {16, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1},
+    {0, -4, 4, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1},
+    {0, 0, 0, -4, 4, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1},
+    {0, 0, 0, 0, 0, -4, 4, 1, -1, -1, 1, 1, -1, -1, 1},
+    {0, 2, 2, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1},
+    {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, -1, -1, 1, 1},
+    {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+    {0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1}};
+const int kM19[15][19] = {
+    {-30, -11, -11, -11, -11, -11, -11, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
+    {12, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+    {0, -4, 4, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
+    {0, 0, 0, -4, 4, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1},
+    {0, 0, 0, 0, 0, -4, 4, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1},
+    {0, 2, 2, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+    {0, -4, -4, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+    {0, 0, 0, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+    {0, 0, 0, -2, -2, 2, 2, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1}};
+
+struct LatticeHost {
+  int Q;
+  int c[19][3];
+  double w[19];
+  explicit LatticeHost(int q) : Q(q) {
+    for (int i = 0; i < Q; ++i) {
+      for (int a = 0; a < 3; ++a) c[i][a] = Q == 15 ? kC15[i][a] : kC19[i][a];
+      w[i] = Q == 15 ? kW15[i] : kW19[i];
+    }
+  }
+  static int inv(int d) { return d == 0 ? 0 : ((d & 1) ? d + 1 : d - 1); }
+};
+
+// Lattice::CalculateFeq, SSE3 variant (Lattice.h:236-321).
+void feq_host(const LatticeHost& L, double rho, const double m[3], double* feq) {
+  const double mx = m[0], my = m[1], mz = m[2];
+  const double three_halves = (3. / 2.) * ((mx * mx + my * my) + mz * mz);
+  const double t1 = rho - three_halves / rho;
+  const double rho_1 = 1. / rho;
+  double nine = (9. / 2.);
+  nine *= rho_1;
+  for (int i = 0; i < L.Q; ++i) {
+    const double e = ((double)L.c[i][0] * mx + (double)L.c[i][1] * my) +
+                     (double)L.c[i][2] * mz;
+    feq[i] = L.w[i] * ((t1 + nine * (e * e)) + 3. * e);
+  }
+}
+
+// util::Dot over Vector3D<double> (Vector3D.h:583-588).
+double dot3(const double* a, const double* b) {
+  return std::inner_product(a, a + 3, b, 0.0);
+}
+
+// util::BesselJ0ComplexArgument (Code/util/Bessel.cc:14-35).
+std::complex<double> bessel_j0(const std::complex<double>& z, double tolSq) {
+  std::complex<double> sum(1.0, 0.0);
+  double fact = 1;
+  std::complex<double> zSqOver4_pow(1.0, 0.0);
+  std::complex<double> term(1.0, 0.0);
+  for (unsigned i = 1; term.real() * term.real() + term.imag() * term.imag() > tolSq;
+       ++i) {
+    fact *= i;
+    zSqOver4_pow *= -0.25 * z * z;
+    term = zSqOver4_pow / (fact * fact);
+    sum += term;
+  }
+  return sum;
+}
+
+struct IoletHost {
+  hlb_iolet cfg;        // normal normalised
+  double min_sim_density = 0;
+  int group = 0;        // 0 inlet, 1 outlet
+  int bc = 0;           // hlb_iolet_bc of its group
+};
+
+// InOutLetCosine::GetDensity (InOutLetCosine.cc:26-43).
+double cosine_density(const IoletHost& io, uint64_t time_step) {
+  const double w = 2.0 * kPi / io.cfg.period;
+  const double target = io.cfg.density_mean +
+                        io.cfg.density_amp * std::cos(w * (double)time_step + io.cfg.phase);
+  if ((int64_t)time_step >= io.cfg.warmup) return target;
+  const double f = ((double)time_step) / ((double)io.cfg.warmup);
+  return f * target + (1. - f) * io.min_sim_density;
+}
+
+// Time-independent factor of InOutLetWomersleyVelocity::GetVelocity at x
+// (InOutLetWomersleyVelocity.cc:21-38):
+//   A = G / (rho * omega) * (1 - J0(i^{3/2} a r / R) / J0(i^{3/2} a)),
+// so that the velocity magnitude is Re(A * exp(i omega t)).
+std::complex<double> womersley_coef(const IoletHost& io, const double* x) {
+  static const std::complex<double> i(0, 1);
+  static const std::complex<double> iPowThreeHalves = pow(i, 1.5);
+  const double displ[3] = {x[0] - io.cfg.position[0], x[1] - io.cfg.position[1],
+                           x[2] - io.cfg.position[2]};
+  const double z = dot3(displ, io.cfg.normal);
+  const double r = std::sqrt(dot3(displ, displ) - z * z);
+  const double omega = 2.0 * kPi / io.cfg.period;
+  const double density = 1.0;
+  const std::complex<double> besselNumer =
+      bessel_j0(iPowThreeHalves * io.cfg.womersley_number * r / io.cfg.radius, 1e-12);
+  const std::complex<double> besselDenom =
+      bessel_j0(iPowThreeHalves * io.cfg.womersley_number, 1e-12);
+  return io.cfg.pressure_gradient_amplitude / (density * omega) *
+         (1.0 - besselNumer / besselDenom);
+}
+
+// 1 - r^2/R^2 of InOutLetParabolicVelocity::GetVelocity (.cc:23-43).
+double parabolic_coef(const IoletHost& io, const double* x) {
+  const double displ[3] = {x[0] - io.cfg.position[0], x[1] - io.cfg.position[1],
+                           x[2] - io.cfg.position[2]};
+  const double z = dot3(displ, io.cfg.normal);
+  const double rSq = (dot3(displ, displ) - z * z) / (io.cfg.radius * io.cfg.radius);
+  return 1. - rSq;
+}
+
+uint64_t spread3(uint64_t v) {
+  uint64_t x = v & 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t n) {
+    count = n;
+    return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1));
+  }
+  hipError_t upload(const std::vector<T>& v) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+  }
+  size_t bytes() const { return p ? sizeof(T) * (count ? count : 1) : 0; }
+};
+
+size_t storage_bytes(int storage) { return storage == 0 ? 8 : storage == 1 ? 4 : 2; }
+
+}  // namespace
+
+struct hlb_solver {
+  hlb_params p{};
+  int Q = 19;
+  int64_t n = 0, np = 0;
+  std::vector<int64_t> perm;      // internal -> caller
+  std::vector<int64_t> internal;  // caller -> internal
+  std::vector<IoletHost> iolets;  // inlets then outlets
+  int n_inlets = 0;
+  // device data
+  DevBuf<uint32_t> src, binfo, b_wall, b_iolet;
+  DevBuf<int32_t> b_io, io_iolet, io_bc, io_site, iolet_kind;
+  DevBuf<float> b_q, iolet_nf;
+  DevBuf<double> io_stash, io_ladd, iolet_n, weights, mrt_c;
+  DevBuf<float> mrt_c_f;
+  DevBuf<int64_t> d_perm;
+  void* state[2] = {nullptr, nullptr};
+  int cur = 0;
+  DevBuf<double> bv[2];
+  double* bv_host[2] = {nullptr, nullptr};
+  hipEvent_t bv_done[2] = {nullptr, nullptr};
+  int bv_cap = 0;
+  int bv_which = 0;
+  DevBuf<int> unstable;
+  // scratch for exports and monitors, allocated on first use
+  DevBuf<double> s_soa, s_aos, rho, u, u_snap, partials, gathered;
+  DevBuf<int64_t> slots;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hlb::DevDomain dom;
+  hlb::StepArgs args;
+  std::vector<int64_t> io_sites_internal;  // per iolet-site record
+  std::vector<int32_t> io_sites_iolet;
+  uint64_t t_ref = 1;
+  bool post_stream = true;
+  bool initialised = false;
+
+  ~hlb_solver() {
+    for (auto& s : state)
+      if (s) (void)hipFree(s);
+    for (auto& h : bv_host)
+      if (h) (void)hipHostFree(h);
+    for (auto& e : bv_done)
+      if (e) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int64_t device_bytes() const {
+    int64_t b = 2 * (int64_t)storage_bytes(p.storage) * Q * np;
+    b += src.bytes() + binfo.bytes() + b_wall.bytes() + b_iolet.bytes() +
+         b_io.bytes() + io_iolet.bytes() + io_bc.bytes() + io_site.bytes() +
+         iolet_kind.bytes() + b_q.bytes() + iolet_nf.bytes() + io_stash.bytes() +
+         io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
+         s_soa.bytes() + s_aos.bytes() + rho.bytes() + u.bytes() + u_snap.bytes();
+    return b;
+  }
+
+  // Boundary values of reference step t (1-indexed) for every iolet: the
+  // Nash density of Get0IndexedTimeStep() = t-1 (BoundaryValues.cc:162-165)
+  // or the Ladd velocity factor at GetTimeStep() = t (LaddIolet.h:52).
+  void boundary_values(uint64_t t, double* out) const {
+    for (size_t k = 0; k < iolets.size(); ++k) {
+      const IoletHost& io = iolets[k];
+      double a = 0, b = 0;
+      if (io.bc == HLB_IOLET_NASH) {
+        a = io.cfg.kind == HLB_IOLET_COSINE ? cosine_density(io, t - 1) : 1.0;
+      } else if (io.cfg.kind == HLB_IOLET_WOMERSLEY) {
+        static const std::complex<double> i(0, 1);
+        const unsigned int ts = (unsigned int)t;  // BoundaryValues::GetTimeStep
+        const double omega = 2.0 * kPi / io.cfg.period;
+        const std::complex<double> e = exp(i * omega * double(ts));
+        a = e.real();
+        b = e.imag();
+      } else {
+        const unsigned int ts = (unsigned int)t;
+        double mx = io.cfg.max_speed;
+        if ((int64_t)ts < io.cfg.warmup) mx *= ts / double(io.cfg.warmup);
+        a = mx;
+      }
+      out[2 * k] = a;
+      out[2 * k + 1] = b;
+    }
+  }
+
+  int ensure_scratch_soa() {
+    if (!s_soa.p) HIP_OR_FAIL(s_soa.alloc((size_t)Q * np));
+    return HLB_OK;
+  }
+  int ensure_moments() {
+    if (!rho.p) HIP_OR_FAIL(rho.alloc((size_t)np));
+    if (!u.p) HIP_OR_FAIL(u.alloc((size_t)3 * np));
+    return HLB_OK;
+  }
+
+  // The reference's fOld as binary64 SoA in s_soa.
+  int post_stream_soa() {
+    int r = ensure_scratch_soa();
+    if (r) return r;
+    int e = hlb::launch_stream_only(dom, args_for_stream(), state[cur], s_soa.p,
+                                    post_stream ? 1 : 0, stream);
+    if (e) return fail(HLB_E_DEVICE, "stream-only kernel launch failed");
+    return HLB_OK;
+  }
+
+  hlb::StepArgs args_for_stream() {
+    hlb::StepArgs a = args;
+    // the pending stream belongs to reference step t_ref - 1
+    if (!post_stream && !iolets.empty()) {
+      std::vector<double> v(2 * iolets.size());
+      boundary_values(t_ref - 1, v.data());
+      // stage through the first bv buffer (synchronously; exports are rare)
+      (void)hipStreamSynchronize(stream);
+      (void)hipMemcpy(bv[0].p, v.data(), sizeof(double) * v.size(),
+                      hipMemcpyHostToDevice);
+      a.bv = bv[0].p;
+    }
+    return a;
+  }
+};
+
+extern "C" {
+
+int hlb_abi_version(void) { return HLB_ABI_VERSION; }
+const char* hlb_last_error(void) { return g_err.c_str(); }
+
+double hlb_tau_from_physical(double dt, double dx, double rho, double eta) {
+  return 0.5 + (dt * eta / rho) / (kCs2 * dx * dx);  // LbmParameters.h:159
+}
+
+static double lattice_pressure(double dt, double dx, double rho) {
+  const double mass = rho * dx * dx * dx;  // UnitConverter.cc:14-22
+  return mass / (dx * dt * dt);
+}
+
+double hlb_pressure_difference_to_lattice(double dp, double dt, double dx, double rho) {
+  return dp * kMmHgToPa / lattice_pressure(dt, dx, rho);
+}
+
+double hlb_pressure_to_lattice(double p, double ref, double dt, double dx, double rho) {
+  return kCs2 + hlb_pressure_difference_to_lattice(p - ref, dt, dx, rho);
+}
+
+double hlb_pressure_gradient_to_lattice(double pg, double dt, double dx, double rho) {
+  return pg * kMmHgToPa * dx / lattice_pressure(dt, dx, rho);
+}
+
+int hlb_create(const hlb_params* params, const hlb_domain* dm,
+               const hlb_iolet* inlets, int32_t n_inlets,
+               const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
+  if (!params || !dm || !out) return fail(HLB_E_INVALID, "null argument");
+  *out = nullptr;
+  const hlb_params& P = *params;
+  if (P.lattice != 15 && P.lattice != 19)
+    return fail(HLB_E_INVALID, "lattice must be HLB_D3Q15 or HLB_D3Q19");
+  if (P.kernel != HLB_LBGK && P.kernel != HLB_MRT)
+    return fail(HLB_E_INVALID, "unknown collision kernel");
+  if (P.wall != HLB_WALL_SBB && P.wall != HLB_WALL_BFL)
+    return fail(HLB_E_INVALID, "unknown wall streamer");
+  for (int bc : {P.inlet_bc, P.outlet_bc})
+    if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD)
+      return fail(HLB_E_INVALID, "unknown iolet streamer");
+  if (P.storage < 0 || P.storage > 2) return fail(HLB_E_INVALID, "unknown storage");
+  if (!(P.tau > 0.0) || !std::isfinite(P.tau)) return fail(HLB_E_INVALID, "tau must be positive");
+  if (P.storage != HLB_STORE_FP64 && !(P.rho0 > 0.0))
+    return fail(HLB_E_INVALID, "rho0 must be positive for reduced-precision storage");
+  const int64_t n = dm->n_sites;
+  if (n <= 0 || n >= (int64_t)0xFFFFFFF0ll) return fail(HLB_E_INVALID, "n_sites out of range");
+  if (!dm->coords || !dm->site_type || !dm->wall_mask || !dm->iolet_mask || !dm->iolet_id)
+    return fail(HLB_E_INVALID, "domain arrays missing");
+  if (P.wall == HLB_WALL_BFL && !dm->cut_distance)
+    return fail(HLB_E_INVALID, "BFL walls need cut distances");
+  if (n_inlets < 0 || n_outlets < 0 || n_inlets + n_outlets > 64)
+    return fail(HLB_E_INVALID, "between 0 and 64 iolets are supported");
+
+  auto s = std::make_unique<hlb_solver>();
+  s->p = P;
+  const int Q = P.lattice;
+  s->Q = Q;
+  s->n = n;
+  s->np = (n + 255) / 256 * 256;
+  const int64_t np = s->np;
+  LatticeHost L(Q);
+  const uint32_t all = (1u << (Q - 1)) - 1u;
+
+  // Iolets: normalised normals (InOutLet::SetNormal, InOutLet.h:160-163) and
+  // the minimum density over all of them (LBM::PrepareBoundaryObjects).
+  double min_density = 1.7976931348623157e308;
+  for (int g = 0; g < 2; ++g) {
+    const hlb_iolet* src = g ? outlets : inlets;
+    const int cnt = g ? n_outlets : n_inlets;
+    if (cnt && !src) return fail(HLB_E_INVALID, "iolet array missing");
+    for (int k = 0; k < cnt; ++k) {
+      IoletHost io;
+      io.cfg = src[k];
+      io.group = g;
+      io.bc = g ? P.outlet_bc : P.inlet_bc;
+      if (io.cfg.kind < 0 || io.cfg.kind > 2) return fail(HLB_E_INVALID, "unknown iolet kind");
+      const double mag = std::sqrt(dot3(src[k].normal, src[k].normal));
+      if (!(mag > 0)) return fail(HLB_E_INVALID, "iolet normal is zero");
+      for (int a = 0; a < 3; ++a) io.cfg.normal[a] = src[k].normal[a] / mag;
+      if (io.bc == HLB_IOLET_LADD && io.cfg.kind == HLB_IOLET_COSINE)
+        return fail(HLB_E_INVALID, "Ladd iolets need a velocity boundary (parabolic or Womersley)");
+      if (io.cfg.kind == HLB_IOLET_COSINE && !(io.cfg.period != 0))
+        return fail(HLB_E_INVALID, "cosine iolet period must be non-zero");
+      if (io.cfg.kind != HLB_IOLET_COSINE && !(io.cfg.radius > 0))
+        return fail(HLB_E_INVALID, "velocity iolet radius must be positive");
+      if (io.cfg.kind == HLB_IOLET_WOMERSLEY && !(io.cfg.period != 0))
+        return fail(HLB_E_INVALID, "Womersley period must be non-zero");
+      const double dmin = io.cfg.kind == HLB_IOLET_COSINE
+                              ? io.cfg.density_mean - io.cfg.density_amp
+                              : 1.0;
+      min_density = std::min(min_density, dmin);
+      s->iolets.push_back(io);
+    }
+  }
+  for (auto& io : s->iolets) io.min_sim_density = min_density;
+  s->n_inlets = n_inlets;
+
+  // Validate per-site data (SiteData semantics, SiteDataBare.cc:23-75).
+  for (int64_t i = 0; i < n; ++i) {
+    const uint32_t t = dm->site_type[i], wm = dm->wall_mask[i], im = dm->iolet_mask[i];
+    if (t < 1 || t > 3) return fail(HLB_E_INVALID, "site " + std::to_string(i) + ": bad site type");
+    if ((wm | im) & ~all)
+      return fail(HLB_E_INVALID, "site " + std::to_string(i) + ": mask has bits beyond Q-1");
+    if ((t == HLB_SITE_FLUID) != (im == 0))
+      return fail(HLB_E_INVALID, "site " + std::to_string(i) +
+                                     ": iolet links must match an INLET/OUTLET site type");
+    if (t != HLB_SITE_FLUID) {
+      const int cnt = t == HLB_SITE_INLET ? n_inlets : n_outlets;
+      if (dm->iolet_id[i] < 0 || dm->iolet_id[i] >= cnt)
+        return fail(HLB_E_INVALID, "site " + std::to_string(i) + ": iolet id out of range");
+    }
+    if (P.wall == HLB_WALL_BFL && wm)
+      for (int d = 1; d < Q; ++d)
+        if ((wm >> (d - 1)) & 1u) {
+          const float q = dm->cut_distance[i * (Q - 1) + d - 1];
+          if (!(q > 0.0f) || !std::isfinite(q))
+            return fail(HLB_E_INVALID, "site " + std::to_string(i) +
+                                           ": BFL wall link needs a positive cut distance");
+        }
+  }
+
+  // Internal order: 8^3 blocks in Morton order, z-fastest inside a block.
+  int32_t lo[3];
+  for (int a = 0; a < 3; ++a) lo[a] = dm->coords[a];
+  for (int64_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], dm->coords[3 * i + a]);
+  auto key_of = [&](int64_t i) {
+    const uint64_t x = (uint64_t)(dm->coords[3 * i] - lo[0]),
+                   y = (uint64_t)(dm->coords[3 * i + 1] - lo[1]),
+                   z = (uint64_t)(dm->coords[3 * i + 2] - lo[2]);
+    const uint64_t oct = (spread3(x >> 3) << 2) | (spread3(y >> 3) << 1) | spread3(z >> 3);
+    return (oct << 9) | (((x & 7) << 6) | ((y & 7) << 3) | (z & 7));
+  };
+  std::vector<std::pair<uint64_t, int64_t>> keys(n);
+  for (int64_t i = 0; i < n; ++i) keys[i] = {key_of(i), i};
+  std::sort(keys.begin(), keys.end());
+  s->perm.resize(n);
+  s->internal.resize(n);
+  for (int64_t o = 0; o < n; ++o) {
+    if (o && keys[o].first == keys[o - 1].first)
+      return fail(HLB_E_INVALID, "two sites share the same coordinates");
+    s->perm[o] = keys[o].second;
+    s->internal[keys[o].second] = o;
+  }
+  // Site lookup by coordinates: block key -> internal index of slot 0..511.
+  std::unordered_map<uint64_t, std::vector<int32_t>> blocks;
+  blocks.reserve(n / 64 + 16);
+  auto bkey = [](int64_t bx, int64_t by, int64_t bz) {
+    return ((uint64_t)(bx & 0x1fffff) << 42) | ((uint64_t)(by & 0x1fffff) << 21) |
+           (uint64_t)(bz & 0x1fffff);
+  };
+  for (int64_t o = 0; o < n; ++o) {
+    const int64_t i = s->perm[o];
+    const int64_t x = dm->coords[3 * i] - lo[0], y = dm->coords[3 * i + 1] - lo[1],
+                  z = dm->coords[3 * i + 2] - lo[2];
+    auto& b = blocks[bkey(x >> 3, y >> 3, z >> 3)];
+    if (b.empty()) b.assign(512, -1);
+    b[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)] = (int32_t)o;
+  }
+  auto lookup = [&](int64_t x, int64_t y, int64_t z) -> int64_t {
+    if (x < 0 || y < 0 || z < 0) return -1;
+    auto it = blocks.find(bkey(x >> 3, y >> 3, z >> 3));
+    if (it == blocks.end()) return -1;
+    return it->second[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)];
+  };
+
+  // Pull table, boundary and iolet records.
+  std::vector<uint32_t> h_src((size_t)(Q - 1) * np, 0), h_binfo(np, 0);
+  std::vector<uint32_t> h_bwall, h_biolet;
+  std::vector<int32_t> h_bio, h_io_iolet, h_io_bc, h_io_site;
+  std::vector<float> h_bq;
+  std::vector<double> h_ladd;
+  for (int64_t o = 0; o < n; ++o) {
+    const int64_t i = s->perm[o];
+    const int64_t x = dm->coords[3 * i] - lo[0], y = dm->coords[3 * i + 1] - lo[1],
+                  z = dm->coords[3 * i + 2] - lo[2];
+    const uint32_t wm = dm->wall_mask[i], im = dm->iolet_mask[i];
+    for (int e = 1; e < Q; ++e) {
+      const int dp = LatticeHost::inv(e);
+      const bool own = ((wm | im) >> (dp - 1)) & 1u;
+      const int64_t j = lookup(x - L.c[e][0], y - L.c[e][1], z - L.c[e][2]);
+      // The source pushes into this slot unless its link e is cut.
+      bool pushes = false;
+      if (j >= 0) {
+        const int64_t ci = s->perm[j];
+        pushes = !(((dm->wall_mask[ci] | dm->iolet_mask[ci]) >> (e - 1)) & 1u);
+      }
+      if (own == pushes) {
+        return fail(HLB_E_INVALID,
+                    "site (" + std::to_string(dm->coords[3 * i]) + "," +
+                        std::to_string(dm->coords[3 * i + 1]) + "," +
+                        std::to_string(dm->coords[3 * i + 2]) + ") direction " +
+                        std::to_string(e) +
+                        (own ? ": slot written both by a cut link and by a fluid neighbour"
+                             : ": slot has no writer (non-fluid neighbour without a cut link)"));
+      }
+      h_src[(size_t)(e - 1) * np + o] = own ? (uint32_t)o : (uint32_t)j;
+    }
+    if (wm | im) {
+      const uint32_t rec = (uint32_t)h_bwall.size();
+      h_binfo[o] = rec + 1;
+      h_bwall.push_back(wm);
+      h_biolet.push_back(im);
+      for (int d = 1; d < Q; ++d)
+        h_bq.push_back(dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1] : -1.0f);
+      if (im) {
+        const int g = dm->site_type[i] == HLB_SITE_OUTLET ? 1 : 0;
+        const int k = (g ? n_inlets : 0) + dm->iolet_id[i];
+        const int32_t io_rec = (int32_t)h_io_iolet.size();
+        h_bio.push_back(io_rec);
+        h_io_iolet.push_back(k);
+        h_io_bc.push_back(s->iolets[k].bc);
+        h_io_site.push_back((int32_t)o);
+        s->io_sites_internal.push_back(o);
+        s->io_sites_iolet.push_back(k);
+        for (int d = 1; d < Q; ++d) {
+          double c0 = 0, c1 = 0;
+          if (s->iolets[k].bc == HLB_IOLET_LADD && ((im >> (d - 1)) & 1u)) {
+            // LaddIoletLink: half-way point x + c_d / 2 (LaddIolet.h:122-125)
+            double half[3];
+            for (int a = 0; a < 3; ++a)
+              half[a] = (double)dm->coords[3 * i + a] + (double)L.c[d][a] * 0.5;
+            if (s->iolets[k].cfg.kind == HLB_IOLET_WOMERSLEY) {
+              const std::complex<double> A = womersley_coef(s->iolets[k], half);
+              c0 = A.real();
+              c1 = A.imag();
+            } else {
+              c0 = parabolic_coef(s->iolets[k], half);
+            }
+          }
+          h_ladd.push_back(c0);
+          h_ladd.push_back(c1);
+        }
+      } else {
+        h_bio.push_back(-1);
+      }
+    }
+  }
+
+  // Device set-up.
+  HIP_OR_FAIL(hipSetDevice(P.device));
+  HIP_OR_FAIL(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIP_OR_FAIL(hipEventCreate(&s->ev0));
+  HIP_OR_FAIL(hipEventCreate(&s->ev1));
+  const size_t sb = storage_bytes(P.storage);
+  for (auto& st : s->state) HIP_OR_FAIL(hipMalloc(&st, sb * Q * np));
+  HIP_OR_FAIL(s->src.upload(h_src));
+  HIP_OR_FAIL(s->binfo.upload(h_binfo));
+  HIP_OR_FAIL(s->b_wall.upload(h_bwall));
+  HIP_OR_FAIL(s->b_iolet.upload(h_biolet));
+  HIP_OR_FAIL(s->b_io.upload(h_bio));
+  HIP_OR_FAIL(s->b_q.upload(h_bq));
+  HIP_OR_FAIL(s->io_iolet.upload(h_io_iolet));
+  HIP_OR_FAIL(s->io_bc.upload(h_io_bc));
+  HIP_OR_FAIL(s->io_site.upload(h_io_site));
+  HIP_OR_FAIL(s->io_ladd.upload(h_ladd));
+  HIP_OR_FAIL(s->io_stash.alloc(h_io_iolet.size()));
+  HIP_OR_FAIL(hipMemset(s->io_stash.p, 0, sizeof(double) * (h_io_iolet.empty() ? 1 : h_io_iolet.size())));
+  std::vector<float> h_nf;
+  std::vector<double> h_n;
+  std::vector<int32_t> h_kind;
+  for (auto& io : s->iolets) {
+    for (int a = 0; a < 3; ++a) {
+      h_nf.push_back((float)io.cfg.normal[a]);
+      h_n.push_back(io.cfg.normal[a]);
+    }
+    h_kind.push_back(io.cfg.kind);
+  }
+  HIP_OR_FAIL(s->iolet_nf.upload(h_nf));
+  HIP_OR_FAIL(s->iolet_n.upload(h_n));
+  HIP_OR_FAIL(s->iolet_kind.upload(h_kind));
+  HIP_OR_FAIL(s->weights.upload(std::vector<double>(L.w, L.w + Q)));
+  HIP_OR_FAIL(s->d_perm.upload(s->perm));
+  HIP_OR_FAIL(s->unstable.alloc(1));
+  HIP_OR_FAIL(hipMemset(s->unstable.p, 0, sizeof(int)));
+  s->bv_cap = 512;
+  const size_t nio = std::max<size_t>(1, s->iolets.size());
+  for (int b = 0; b < 2; ++b) {
+    HIP_OR_FAIL(s->bv[b].alloc((size_t)s->bv_cap * 2 * nio));
+    HIP_OR_FAIL(hipHostMalloc((void**)&s->bv_host[b], sizeof(double) * s->bv_cap * 2 * nio));
+    HIP_OR_FAIL(hipEventCreateWithFlags(&s->bv_done[b], hipEventDisableTiming));
+    HIP_OR_FAIL(hipEventRecord(s->bv_done[b], s->stream));
+  }
+
+  hlb::DevDomain& D = s->dom;
+  D.n = n;
+  D.np = np;
+  D.src = s->src.p;
+  D.binfo = s->binfo.p;
+  D.nb = (int64_t)h_bwall.size();
+  D.b_wall = s->b_wall.p;
+  D.b_iolet = s->b_iolet.p;
+  D.b_io = s->b_io.p;
+  D.b_q = s->b_q.p;
+  D.ni = (int64_t)h_io_iolet.size();
+  D.io_iolet = s->io_iolet.p;
+  D.io_bc = s->io_bc.p;
+  D.io_site = s->io_site.p;
+  D.io_stash = s->io_stash.p;
+  D.io_ladd = s->io_ladd.p;
+  D.n_iolets = (int32_t)s->iolets.size();
+  D.iolet_nf = s->iolet_nf.p;
+  D.iolet_n = s->iolet_n.p;
+  D.iolet_kind = s->iolet_kind.p;
+
+  hlb::StepArgs& A = s->args;
+  A.Q = Q;
+  A.kernel = P.kernel;
+  A.wall = P.wall;
+  A.storage = P.storage;
+  A.omega = -1.0 / P.tau;  // LbmParameters::CalcDerivedParams
+  A.omega_f = (float)A.omega;
+  A.rho0 = P.storage == HLB_STORE_FP64 ? 0.0 : P.rho0;
+  A.unstable = s->unstable.p;
+  if (P.kernel == HLB_MRT) {
+    // collisionMatrixDiagonals * normalisedReducedMomentBasis (MRT.h:96-102,139-152)
+    const int nm = Q == 15 ? 11 : 15;
+    double S[15];
+    if (Q == 15) {
+      const double v[11] = {1.6, 1.2, 1.6, 1.6, 1.6, 1.0 / P.tau, 1.0 / P.tau,
+                            1.0 / P.tau, 1.0 / P.tau, 1.0 / P.tau, 1.2};
+      std::copy(v, v + 11, S);
+    } else {
+      const double v[15] = {1.19, 1.4, 1.2, 1.2, 1.2, 1.0 / P.tau, 1.4, 1.0 / P.tau,
+                            1.4, 1.0 / P.tau, 1.0 / P.tau, 1.0 / P.tau, 1.98, 1.98, 1.98};
+      std::copy(v, v + 15, S);
+    }
+    std::vector<double> c((size_t)nm * Q);
+    std::vector<float> cf((size_t)nm * Q);
+    for (int k = 0; k < nm; ++k) {
+      double diag = 0.0;
+      for (int i = 0; i < Q; ++i) {
+        const double m = Q == 15 ? kM15[k][i] : kM19[k][i];
+        diag = diag + m * m;
+      }
+      for (int d = 0; d < Q; ++d) {
+        const double m = Q == 15 ? kM15[k][d] : kM19[k][d];
+        const double N = m / diag;
+        c[k * Q + d] = S[k] * N;
+        cf[k * Q + d] = (float)c[k * Q + d];
+      }
+    }
+    HIP_OR_FAIL(s->mrt_c.upload(c));
+    HIP_OR_FAIL(s->mrt_c_f.upload(cf));
+    A.mrt_c = s->mrt_c.p;
+    A.mrt_c_f = s->mrt_c_f.p;
+  }
+  s->initialised = false;
+  *out = s.release();
+  return HLB_OK;
+}
+
+void hlb_destroy(hlb_solver* s) { delete s; }
+
+int hlb_set_equilibrium(hlb_solver* s, double density, const double momentum[3]) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  LatticeHost L(s->Q);
+  double m[3] = {0, 0, 0};
+  if (momentum) std::copy(momentum, momentum + 3, m);
+  double feq[19];
+  feq_host(L, density, m, feq);
+  DevBuf<double> vals;
+  HIP_OR_FAIL(vals.upload(std::vector<double>(feq, feq + s->Q)));
+  if (hlb::launch_fill(s->args, s->n, s->np, vals.p, s->state[s->cur], s->stream))
+    return fail(HLB_E_DEVICE, "fill kernel launch failed");
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  s->post_stream = true;
+  s->t_ref = 1;
+  s->initialised = true;
+  return HLB_OK;
+}
+
+int hlb_set_time(hlb_solver* s, uint64_t t) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  if (!s->post_stream) return fail(HLB_E_STATE, "time can only be set before stepping");
+  if (t < 1) return fail(HLB_E_INVALID, "time steps start at 1");
+  s->t_ref = t;
+  return HLB_OK;
+}
+
+uint64_t hlb_get_time(const hlb_solver* s) { return s ? s->t_ref : 0; }
+
+int hlb_step(hlb_solver* s, int64_t nsteps) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  if (nsteps < 0) return fail(HLB_E_INVALID, "negative step count");
+  while (nsteps > 0) {
+    if (s->post_stream) {
+      // Collision of step t_ref from the post-stream state; its stream is
+      // carried out by the next pull.
+      if (hlb::launch_collide_only(s->dom, s->args, s->state[s->cur],
+                                   s->state[s->cur ^ 1], s->stream))
+        return fail(HLB_E_DEVICE, "collide kernel launch failed");
+      s->cur ^= 1;
+      s->post_stream = false;
+      s->t_ref += 1;
+      --nsteps;
+      continue;
+    }
+    const int64_t batch = std::min<int64_t>(nsteps, s->bv_cap);
+    const size_t nio = s->iolets.size();
+    const int b = s->bv_which;
+    s->bv_which ^= 1;
+    if (nio) {
+      HIP_OR_FAIL(hipEventSynchronize(s->bv_done[b]));
+      for (int64_t k = 0; k < batch; ++k)
+        s->boundary_values(s->t_ref - 1 + k, s->bv_host[b] + k * 2 * nio);
+      HIP_OR_FAIL(hipMemcpyAsync(s->bv[b].p, s->bv_host[b], sizeof(double) * batch * 2 * nio,
+                                 hipMemcpyHostToDevice, s->stream));
+    }
+    for (int64_t k = 0; k < batch; ++k) {
+      hlb::StepArgs a = s->args;
+      a.bv = nio ? s->bv[b].p + k * 2 * nio : nullptr;
+      if (hlb::launch_step(s->dom, a, s->state[s->cur], s->state[s->cur ^ 1], s->stream))
+        return fail(HLB_E_DEVICE, "step kernel launch failed");
+      s->cur ^= 1;
+    }
+    HIP_OR_FAIL(hipEventRecord(s->bv_done[b], s->stream));
+    s->t_ref += batch;
+    nsteps -= batch;
+  }
+  return HLB_OK;
+}
+
+int hlb_time_steps(hlb_solver* s, int64_t n, double* ms) {
+  if (!s || !ms) return fail(HLB_E_INVALID, "null argument");
+  HIP_OR_FAIL(hipEventRecord(s->ev0, s->stream));
+  int r = hlb_step(s, n);
+  if (r) return r;
+  HIP_OR_FAIL(hipEventRecord(s->ev1, s->stream));
+  HIP_OR_FAIL(hipEventSynchronize(s->ev1));
+  float f = 0;
+  HIP_OR_FAIL(hipEventElapsedTime(&f, s->ev0, s->ev1));
+  *ms = f;
+  return HLB_OK;
+}
+
+int hlb_get_distributions(hlb_solver* s, double* f_out) {
+  if (!s || !f_out) return fail(HLB_E_INVALID, "null argument");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  int r = s->post_stream_soa();
+  if (r) return r;
+  if (!s->s_aos.p) HIP_OR_FAIL(s->s_aos.alloc((size_t)s->Q * s->n));
+  if (hlb::launch_soa_to_aos(s->Q, s->n, s->np, s->s_soa.p, s->d_perm.p, s->s_aos.p, s->stream))
+    return fail(HLB_E_DEVICE, "permute kernel launch failed");
+  HIP_OR_FAIL(hipMemcpyAsync(f_out, s->s_aos.p, sizeof(double) * s->Q * s->n,
+                             hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  return HLB_OK;
+}
+
+int hlb_set_distributions(hlb_solver* s, const double* f_in) {
+  if (!s || !f_in) return fail(HLB_E_INVALID, "null argument");
+  if (!s->s_aos.p) HIP_OR_FAIL(s->s_aos.alloc((size_t)s->Q * s->n));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  HIP_OR_FAIL(hipMemcpy(s->s_aos.p, f_in, sizeof(double) * s->Q * s->n, hipMemcpyHostToDevice));
+  if (hlb::launch_aos_to_state(s->args, s->n, s->np, s->s_aos.p, s->d_perm.p,
+                               s->state[s->cur], s->stream))
+    return fail(HLB_E_DEVICE, "import kernel launch failed");
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  s->post_stream = true;
+  s->initialised = true;
+  return HLB_OK;
+}
+
+static int compute_moments(hlb_solver* s) {
+  int r = s->post_stream_soa();
+  if (r) return r;
+  r = s->ensure_moments();
+  if (r) return r;
+  if (hlb::launch_moments(s->Q, s->n, s->np, s->s_soa.p, s->rho.p, s->u.p, s->stream))
+    return fail(HLB_E_DEVICE, "moments kernel launch failed");
+  return HLB_OK;
+}
+
+int hlb_get_macroscopic(hlb_solver* s, double* density, double* velocity) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  int r = compute_moments(s);
+  if (r) return r;
+  std::vector<double> rho(s->np), u(3 * s->np);
+  HIP_OR_FAIL(hipMemcpyAsync(rho.data(), s->rho.p, sizeof(double) * s->np,
+                             hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(u.data(), s->u.p, sizeof(double) * 3 * s->np,
+                             hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  for (int64_t o = 0; o < s->n; ++o) {
+    const int64_t c = s->perm[o];
+    if (density) density[c] = rho[o];
+    if (velocity)
+      for (int a = 0; a < 3; ++a) velocity[3 * c + a] = u[(size_t)a * s->np + o];
+  }
+  return HLB_OK;
+}
+
+int hlb_reduce(hlb_solver* s, hlb_monitor* out) {
+  if (!s || !out) return fail(HLB_E_INVALID, "null argument");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  int r = compute_moments(s);
+  if (r) return r;
+  const int nb = 1024;
+  if (!s->partials.p) HIP_OR_FAIL(s->partials.alloc(5 * nb));
+  if (hlb::launch_reduce(s->Q, s->n, s->np, s->s_soa.p, s->rho.p, s->u.p, s->partials.p, nb,
+                         s->stream))
+    return fail(HLB_E_DEVICE, "reduction kernel launch failed");
+  std::vector<double> part(5 * nb);
+  HIP_OR_FAIL(hipMemcpyAsync(part.data(), s->partials.p, sizeof(double) * 5 * nb,
+                             hipMemcpyDeviceToHost, s->stream));
+  // Iolet fluxes: gather rho and u at the iolet sites.
+  const int64_t ni = (int64_t)s->io_sites_internal.size();
+  std::vector<double> g(4 * std::max<int64_t>(ni, 1));
+  if (ni) {
+    std::vector<int64_t> slots(4 * ni);
+    for (int64_t k = 0; k < ni; ++k) {
+      const int64_t o = s->io_sites_internal[k];
+      slots[k] = o;                         // rho
+      slots[ni + k] = o;                    // u_x
+      slots[2 * ni + k] = s->np + o;        // u_y
+      slots[3 * ni + k] = 2 * s->np + o;    // u_z
+    }
+    DevBuf<int64_t> d_slots;
+    DevBuf<double> d_g;
+    HIP_OR_FAIL(d_slots.upload(slots));
+    HIP_OR_FAIL(d_g.alloc(4 * ni));
+    if (hlb::launch_gather_pairs(ni, s->np, d_slots.p, s->rho.p, 0, 0.0, nullptr, d_g.p, s->stream) ||
+        hlb::launch_gather_pairs(3 * ni, s->np, d_slots.p + ni, s->u.p, 0, 0.0, nullptr,
+                                 d_g.p + ni, s->stream))
+      return fail(HLB_E_DEVICE, "gather kernel launch failed");
+    HIP_OR_FAIL(hipMemcpyAsync(g.data(), d_g.p, sizeof(double) * 4 * ni, hipMemcpyDeviceToHost,
+                               s->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  }
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  hlb_monitor m{};
+  m.density_min = 1.7976931348623157e308;
+  m.density_max = -1.7976931348623157e308;
+  bool bad = false;
+  for (int b = 0; b < nb; ++b) {
+    const double* p = &part[5 * b];
+    m.mass += p[0];
+    m.density_min = std::min(m.density_min, p[1]);
+    m.density_max = std::max(m.density_max, p[2]);
+    m.velocity_max = std::max(m.velocity_max, p[3]);
+    bad = bad || p[4] != 0.0;
+  }
+  m.stable = bad ? 0 : 1;
+  m.n_iolets = (int32_t)s->iolets.size();
+  for (int64_t k = 0; k < ni; ++k) {
+    const int io = s->io_sites_iolet[k];
+    const double* nrm = s->iolets[io].cfg.normal;
+    const double un = ((0.0 + g[ni + k] * nrm[0]) + g[2 * ni + k] * nrm[1]) + g[3 * ni + k] * nrm[2];
+    m.iolet_flux[io] += g[k] * un;
+  }
+  *out = m;
+  return HLB_OK;
+}
+
+int hlb_snapshot_velocity(hlb_solver* s) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  int r = compute_moments(s);
+  if (r) return r;
+  if (!s->u_snap.p) HIP_OR_FAIL(s->u_snap.alloc(3 * s->np));
+  HIP_OR_FAIL(hipMemcpyAsync(s->u_snap.p, s->u.p, sizeof(double) * 3 * s->np,
+                             hipMemcpyDeviceToDevice, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  return HLB_OK;
+}
+
+int hlb_velocity_change(hlb_solver* s, double ref, double* out) {
+  if (!s || !out) return fail(HLB_E_INVALID, "null argument");
+  if (!s->u_snap.p) return fail(HLB_E_STATE, "no velocity snapshot");
+  if (!(ref > 0)) return fail(HLB_E_INVALID, "reference speed must be positive");
+  int r = compute_moments(s);
+  if (r) return r;
+  const int nb = 1024;
+  if (!s->partials.p) HIP_OR_FAIL(s->partials.alloc(5 * nb));
+  if (hlb::launch_velocity_change(s->n, s->np, s->u.p, s->u_snap.p, 1.0 / ref, s->partials.p,
+                                  nb, s->stream))
+    return fail(HLB_E_DEVICE, "velocity change kernel launch failed");
+  std::vector<double> part(nb);
+  HIP_OR_FAIL(hipMemcpyAsync(part.data(), s->partials.p, sizeof(double) * nb,
+                             hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  double m = 0.0;
+  for (double v : part) m = std::max(m, v);
+  *out = m;
+  return HLB_OK;
+}
+
+int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count, double* host_buffer) {
+  if (!s || (count && (!pairs || !host_buffer))) return fail(HLB_E_INVALID, "null argument");
+  if (s->post_stream)
+    return fail(HLB_E_STATE, "no post-collision state yet (step at least once)");
+  if (count == 0) return HLB_OK;
+  std::vector<int64_t> slots(count);
+  for (int64_t k = 0; k < count; ++k) {
+    const int64_t site = pairs[2 * k], d = pairs[2 * k + 1];
+    if (site < 0 || site >= s->n || d < 0 || d >= s->Q)
+      return fail(HLB_E_INVALID, "halo pair out of range");
+    slots[k] = d * s->np + s->internal[site];
+  }
+  DevBuf<int64_t> d_slots;
+  DevBuf<double> d_out;
+  HIP_OR_FAIL(d_slots.upload(slots));
+  HIP_OR_FAIL(d_out.alloc(count));
+  if (hlb::launch_gather_pairs(count, s->np, d_slots.p, s->state[s->cur], s->p.storage,
+                               s->args.rho0, s->weights.p, d_out.p, s->stream))
+    return fail(HLB_E_DEVICE, "halo pack kernel launch failed");
+  HIP_OR_FAIL(hipMemcpyAsync(host_buffer, d_out.p, sizeof(double) * count,
+                             hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  return HLB_OK;
+}
+
+int64_t hlb_site_count(const hlb_solver* s) { return s ? s->n : -1; }
+int32_t hlb_lattice_q(const hlb_solver* s) { return s ? s->Q : -1; }
+int64_t hlb_device_bytes(const hlb_solver* s) { return s ? s->device_bytes() : -1; }
+void* hlb_stream(const hlb_solver* s) { return s ? (void*)s->stream : nullptr; }
+
+int hlb_synchronize(hlb_solver* s) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  return HLB_OK;
+}
+
+}  // extern "C"

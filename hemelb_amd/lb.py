@@ -1,0 +1,335 @@
+"""Host-side mirror of the reference's ``hemelb::lb`` API for the timestep.
+
+Names and argument meanings follow Code/lb: ``Traits`` (Code/Traits.h:17-39)
+selects lattice, kernel, wall and iolet streamers; ``LbmParameters``
+(Code/lb/LbmParameters.h:32-167) derives tau and omega; the iolet classes
+mirror Code/lb/iolets/InOutLet*.h; ``LBM`` is the driver that
+``SimulationMaster`` steps (Code/lb/lb.h:30-142).  All computation happens in
+``libhemelb_amd.so`` on the GPU; this module only marshals arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .geometry import Domain
+
+
+# ---------------------------------------------------------------------------
+# Lattices (Code/lb/lattices/D3Q15.h:15-56, D3Q19.h:14-49)
+# ---------------------------------------------------------------------------
+class _Lattice:
+    NUMVECTORS: int
+    VECTORS: np.ndarray
+    EQMWEIGHTS: np.ndarray
+
+    @classmethod
+    def inverse(cls) -> np.ndarray:
+        """INVERSEDIRECTIONS (Lattice.h:49-69)."""
+        inv = np.full(cls.NUMVECTORS, -1, np.int32)
+        for i in range(cls.NUMVECTORS):
+            for j in range(i, cls.NUMVECTORS):
+                if (cls.VECTORS[i] == -cls.VECTORS[j]).all():
+                    inv[i], inv[j] = j, i
+        return inv
+
+
+class D3Q15(_Lattice):
+    NUMVECTORS = 15
+    VECTORS = np.array([(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0),
+                        (0, 0, 1), (0, 0, -1), (1, 1, 1), (-1, -1, -1), (1, 1, -1),
+                        (-1, -1, 1), (1, -1, 1), (-1, 1, -1), (1, -1, -1), (-1, 1, 1)],
+                       np.int32)
+    EQMWEIGHTS = np.array([2.0 / 9.0] + [1.0 / 9.0] * 6 + [1.0 / 72.0] * 8)
+
+
+class D3Q19(_Lattice):
+    NUMVECTORS = 19
+    VECTORS = np.array([(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0),
+                        (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0), (1, -1, 0),
+                        (-1, 1, 0), (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1),
+                        (0, 1, 1), (0, -1, -1), (0, 1, -1), (0, -1, 1)], np.int32)
+    EQMWEIGHTS = np.array([1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12)
+
+
+LATTICES = {15: D3Q15, 19: D3Q19}
+
+
+# Collision kernels and streamers: tags that select the compiled variants.
+class LBGK:  # Code/lb/kernels/LBGK.h
+    CODE = N.HLB_LBGK
+
+
+class MRT:   # Code/lb/kernels/MRT.h with the d'Humieres basis of the lattice
+    CODE = N.HLB_MRT
+
+
+class SimpleBounceBack:           # Code/lb/streamers/SimpleBounceBack.h
+    CODE = N.HLB_WALL_SBB
+
+
+class BouzidiFirdaousLallemand:   # Code/lb/streamers/BouzidiFirdaousLallemand.h
+    CODE = N.HLB_WALL_BFL
+
+
+class NashZerothOrderPressure:    # Code/lb/streamers/NashZerothOrderPressure.h
+    CODE = N.HLB_IOLET_NASH
+
+
+class LaddIolet:                  # Code/lb/streamers/LaddIolet.h
+    CODE = N.HLB_IOLET_LADD
+
+
+@dataclass
+class Traits:
+    """Compile-time choices of the reference build (Code/Traits.h:17-39,
+    CMake/HemeLbOptions.cmake HEMELB_LATTICE/KERNEL/WALL_BOUNDARY/
+    INLET_BOUNDARY/OUTLET_BOUNDARY); defaults are the reference defaults."""
+    Lattice: type = D3Q15
+    Kernel: type = LBGK
+    WallStreamer: type = SimpleBounceBack
+    InletStreamer: type = NashZerothOrderPressure
+    OutletStreamer: type = NashZerothOrderPressure
+
+
+# ---------------------------------------------------------------------------
+# Parameters and units
+# ---------------------------------------------------------------------------
+Cs2 = 1.0 / 3.0                 # constants.h:41
+mmHg_TO_PASCAL = 133.3223874    # constants.h:20
+
+
+class LbmParameters:
+    """tau = 0.5 + (dt*eta/rho)/(Cs2*dx^2), omega = -1/tau (LbmParameters.h:158-163)."""
+
+    def __init__(self, time_step_s=1.0, voxel_size_m=1.0, fluid_density_kgm3=1000.0,
+                 viscosity_pas=0.004, tau=None):
+        self.time_step = time_step_s
+        self.voxel_size = voxel_size_m
+        self.fluid_density = fluid_density_kgm3
+        self.eta = viscosity_pas
+        self.tau = (N.lib().hlb_tau_from_physical(time_step_s, voxel_size_m,
+                                                  fluid_density_kgm3, viscosity_pas)
+                    if tau is None else float(tau))
+
+    def GetTau(self) -> float:
+        return self.tau
+
+    def GetOmega(self) -> float:
+        return -1.0 / self.tau
+
+
+class UnitConverter:
+    """util::UnitConverter pressure conversions (Code/util/UnitConverter.cc:14-48)."""
+
+    def __init__(self, time_step_s, voxel_size_m, fluid_density_kgm3=1000.0,
+                 reference_pressure_mmHg=0.0):
+        self.dt, self.dx, self.rho = time_step_s, voxel_size_m, fluid_density_kgm3
+        self.ref = reference_pressure_mmHg
+
+    def ConvertPressureToLatticeUnits(self, p_mmhg):
+        return N.lib().hlb_pressure_to_lattice(p_mmhg, self.ref, self.dt, self.dx, self.rho)
+
+    def ConvertPressureDifferenceToLatticeUnits(self, dp_mmhg):
+        return N.lib().hlb_pressure_difference_to_lattice(dp_mmhg, self.dt, self.dx, self.rho)
+
+    def ConvertPressureGradientToLatticeUnits(self, pg):
+        return N.lib().hlb_pressure_gradient_to_lattice(pg, self.dt, self.dx, self.rho)
+
+    def ConvertTimeToLatticeUnits(self, t_s):
+        return t_s / self.dt
+
+    def ConvertDistanceToLatticeUnits(self, x_m):
+        return x_m / self.dx
+
+    def ConvertSpeedToLatticeUnits(self, v):
+        return v / (self.dx / self.dt)
+
+
+# ---------------------------------------------------------------------------
+# Iolets (Code/lb/iolets)
+# ---------------------------------------------------------------------------
+@dataclass
+class InOutLet:
+    position: tuple = (0.0, 0.0, 0.0)   # lattice units
+    normal: tuple = (0.0, 0.0, 1.0)     # normalised by the library
+
+    def _fill(self, s: N.hlb_iolet) -> None:
+        s.position[:] = list(map(float, self.position))
+        s.normal[:] = list(map(float, self.normal))
+
+
+@dataclass
+class InOutLetCosine(InOutLet):
+    """Density mean + amp*cos(2 pi t/period + phase) (InOutLetCosine.cc:26-43)."""
+    density_mean: float = 1.0
+    density_amp: float = 0.0
+    phase: float = 0.0
+    period: float = 1.0
+    warmup: int = 0
+
+    def to_c(self) -> N.hlb_iolet:
+        s = N.hlb_iolet()
+        s.kind = N.HLB_IOLET_COSINE
+        self._fill(s)
+        s.density_mean, s.density_amp = self.density_mean, self.density_amp
+        s.phase, s.period, s.warmup = self.phase, self.period, int(self.warmup)
+        return s
+
+    def GetDensity(self, time_step: int, minimum_simulation_density: float = 0.0) -> float:
+        w = 2.0 * math.pi / self.period
+        target = self.density_mean + self.density_amp * math.cos(w * time_step + self.phase)
+        if time_step >= self.warmup:
+            return target
+        f = time_step / self.warmup
+        return f * target + (1.0 - f) * minimum_simulation_density
+
+
+@dataclass
+class InOutLetParabolicVelocity(InOutLet):
+    radius: float = 1.0
+    max_speed: float = 0.0
+    warmup: int = 0
+
+    def to_c(self) -> N.hlb_iolet:
+        s = N.hlb_iolet()
+        s.kind = N.HLB_IOLET_PARABOLIC
+        self._fill(s)
+        s.radius, s.max_speed, s.warmup = self.radius, self.max_speed, int(self.warmup)
+        return s
+
+
+@dataclass
+class InOutLetWomersleyVelocity(InOutLet):
+    radius: float = 1.0
+    pressure_gradient_amplitude: float = 0.0
+    period: float = 1.0
+    womersley_number: float = 1.0
+
+    def to_c(self) -> N.hlb_iolet:
+        s = N.hlb_iolet()
+        s.kind = N.HLB_IOLET_WOMERSLEY
+        self._fill(s)
+        s.radius = self.radius
+        s.pressure_gradient_amplitude = self.pressure_gradient_amplitude
+        s.period = self.period
+        s.womersley_number = self.womersley_number
+        return s
+
+
+def _iolet_array(lst):
+    arr = (N.hlb_iolet * max(1, len(lst)))()
+    for i, io in enumerate(lst):
+        arr[i] = io.to_c()
+    return arr
+
+
+STORAGE = {"fp64": N.HLB_STORE_FP64, "fp32": N.HLB_STORE_FP32, "fp16": N.HLB_STORE_FP16}
+
+
+class LBM:
+    """The LB actor of one rank on one GPU (Code/lb/lb.h:30-142)."""
+
+    def __init__(self, traits: Traits, domain: Domain, params: LbmParameters,
+                 inlets=(), outlets=(), storage: str = "fp64", rho0: float = 1.0,
+                 device: int = 0):
+        if domain.Q != traits.Lattice.NUMVECTORS:
+            raise ValueError("domain was projected for a different lattice")
+        self.traits, self.domain, self.params = traits, domain, params
+        self.Q = domain.Q
+        p = N.hlb_params()
+        p.lattice = self.Q
+        p.kernel = traits.Kernel.CODE
+        p.wall = traits.WallStreamer.CODE
+        p.inlet_bc = traits.InletStreamer.CODE
+        p.outlet_bc = traits.OutletStreamer.CODE
+        p.storage = STORAGE[storage]
+        p.tau = params.GetTau()
+        p.rho0 = rho0
+        p.device = device
+        self._inlets, self._outlets = list(inlets), list(outlets)
+        self._cdom = domain.as_c()
+        h = C.c_void_p()
+        N.check(N.lib().hlb_create(C.byref(p), C.byref(self._cdom),
+                                   _iolet_array(self._inlets), len(self._inlets),
+                                   _iolet_array(self._outlets), len(self._outlets),
+                                   C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            N.lib().hlb_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def close(self):
+        self.__del__()
+
+    # InitialCondition / SimulationState
+    def SetInitialConditions(self, density: float, momentum=(0.0, 0.0, 0.0)) -> None:
+        m = np.asarray(momentum, np.float64)
+        N.check(N.lib().hlb_set_equilibrium(self._h, float(density),
+                                            m.ctypes.data_as(C.c_void_p)))
+
+    def SetTimeStep(self, t: int) -> None:
+        N.check(N.lib().hlb_set_time(self._h, int(t)))
+
+    def GetTimeStep(self) -> int:
+        return int(N.lib().hlb_get_time(self._h))
+
+    def step(self, n: int = 1) -> None:
+        N.check(N.lib().hlb_step(self._h, int(n)))
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double()
+        N.check(N.lib().hlb_time_steps(self._h, int(n), C.byref(ms)))
+        return ms.value
+
+    # FieldData export/import
+    def get_distributions(self) -> np.ndarray:
+        f = np.empty((self.domain.n, self.Q), np.float64)
+        N.check(N.lib().hlb_get_distributions(self._h, f.ctypes.data_as(C.c_void_p)))
+        return f
+
+    def set_distributions(self, f: np.ndarray) -> None:
+        f = np.ascontiguousarray(f, np.float64)
+        assert f.shape == (self.domain.n, self.Q)
+        N.check(N.lib().hlb_set_distributions(self._h, f.ctypes.data_as(C.c_void_p)))
+
+    def macroscopic(self):
+        rho = np.empty(self.domain.n, np.float64)
+        u = np.empty((self.domain.n, 3), np.float64)
+        N.check(N.lib().hlb_get_macroscopic(self._h, rho.ctypes.data_as(C.c_void_p),
+                                            u.ctypes.data_as(C.c_void_p)))
+        return rho, u
+
+    def monitor(self) -> dict:
+        m = N.hlb_monitor()
+        N.check(N.lib().hlb_reduce(self._h, C.byref(m)))
+        return {"mass": m.mass, "density_min": m.density_min,
+                "density_max": m.density_max, "velocity_max": m.velocity_max,
+                "stable": bool(m.stable), "iolet_flux": list(m.iolet_flux[:m.n_iolets])}
+
+    def snapshot_velocity(self) -> None:
+        N.check(N.lib().hlb_snapshot_velocity(self._h))
+
+    def velocity_change(self, reference_speed: float) -> float:
+        v = C.c_double()
+        N.check(N.lib().hlb_velocity_change(self._h, reference_speed, C.byref(v)))
+        return v.value
+
+    def halo_pack(self, pairs: np.ndarray) -> np.ndarray:
+        pairs = np.ascontiguousarray(pairs, np.int64)
+        out = np.empty(pairs.shape[0], np.float64)
+        N.check(N.lib().hlb_halo_pack(self._h, pairs.ctypes.data_as(C.c_void_p),
+                                      pairs.shape[0], out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def device_bytes(self) -> int:
+        return int(N.lib().hlb_device_bytes(self._h))
+
+    def synchronize(self) -> None:
+        N.check(N.lib().hlb_synchronize(self._h))

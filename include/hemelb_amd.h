@@ -1,0 +1,215 @@
+/*
+ * hemelb_amd.h -- C ABI of the MI355X lattice-Boltzmann timestep.
+ *
+ * This library replaces the reference's compile-time LB hot path: the
+ * hemelb::lb::LBM<Traits> IteratedAction (Code/lb/lb.h:30-142,
+ * Code/lb/lb.hpp:74-309) with its collision kernels (LBGK, MRT), streamers
+ * (bulk, simple bounce-back, BFL, Nash zeroth-order pressure, Ladd) and
+ * lattices (D3Q15, D3Q19), the FieldData distributions it updates
+ * (Code/geometry/FieldData.h:27-214) and the BoundaryValues / InOutLet
+ * objects that feed the iolet streamers (Code/lb/iolets/BoundaryValues.cc,
+ * InOutLetCosine.cc, InOutLetParabolicVelocity.cc,
+ * InOutLetWomersleyVelocity.cc).
+ *
+ * Conventions
+ *  - Plain C types only; arrays are caller-owned host memory unless stated.
+ *  - Every function returns HLB_OK (0) or a negative HLB_E* code; no C++
+ *    exception crosses the boundary.  hlb_last_error() gives a message for
+ *    the last failure on the calling thread.
+ *  - A solver is used from one host thread at a time; all of its device work
+ *    is ordered on its own HIP stream.
+ *  - Site arrays describe the reference's geometry::Domain for one rank
+ *    (Code/geometry/Domain.h:53-531): n sites in the caller's local order,
+ *    with wall/iolet link bitmasks (bit d-1 for lattice direction d,
+ *    Code/geometry/SiteDataBare.cc:147-163), one iolet id per site and cut
+ *    distances per lattice direction (Domain::GetCutDistance, Domain.h:405).
+ *  - Distributions crossing the ABI are in the reference's FieldData layout:
+ *    double f[n][Q] in the caller's site order.
+ */
+#ifndef HEMELB_AMD_H
+#define HEMELB_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HLB_ABI_VERSION 1
+
+enum hlb_status {
+  HLB_OK = 0,
+  HLB_E_INVALID = -1,   /* invalid argument or inconsistent geometry */
+  HLB_E_DEVICE = -2,    /* HIP runtime failure */
+  HLB_E_NOMEM = -3,     /* allocation failure */
+  HLB_E_STATE = -4      /* call not valid in the solver's current state */
+};
+
+/* Lattices (Code/lb/lattices/D3Q15.h, D3Q19.h). */
+enum hlb_lattice { HLB_D3Q15 = 15, HLB_D3Q19 = 19 };
+/* Collision kernels (Code/lb/kernels/LBGK.h, MRT.h with the d'Humieres
+ * bases DHumieresD3Q15MRTBasis / DHumieresD3Q19MRTBasis). */
+enum hlb_kernel { HLB_LBGK = 0, HLB_MRT = 1 };
+/* Wall link streamers (Code/lb/streamers/SimpleBounceBack.h,
+ * BouzidiFirdaousLallemand.h). */
+enum hlb_wall { HLB_WALL_SBB = 0, HLB_WALL_BFL = 1 };
+/* Iolet link streamers (Code/lb/streamers/NashZerothOrderPressure.h,
+ * LaddIolet.h). */
+enum hlb_iolet_bc { HLB_IOLET_NASH = 0, HLB_IOLET_LADD = 1 };
+/* Boundary-value objects (Code/lb/iolets/InOutLetCosine.h,
+ * InOutLetParabolicVelocity.h, InOutLetWomersleyVelocity.h). */
+enum hlb_iolet_kind { HLB_IOLET_COSINE = 0, HLB_IOLET_PARABOLIC = 1,
+                      HLB_IOLET_WOMERSLEY = 2 };
+/* Site types (geometry::SiteType). */
+enum hlb_site_type { HLB_SITE_FLUID = 1, HLB_SITE_INLET = 2, HLB_SITE_OUTLET = 3 };
+/* Storage of the distributions in HBM.  HLB_STORE_FP64 is the plain method
+ * (all arithmetic in binary64, bit-compatible with the reference).  FP32 and
+ * FP16 store f_i - w_i*rho0 and collide in binary32; iolet boundary values
+ * and all reductions stay binary64. */
+enum hlb_storage { HLB_STORE_FP64 = 0, HLB_STORE_FP32 = 1, HLB_STORE_FP16 = 2 };
+
+typedef struct hlb_solver hlb_solver;
+
+typedef struct {
+  int32_t lattice;     /* hlb_lattice */
+  int32_t kernel;      /* hlb_kernel */
+  int32_t wall;        /* hlb_wall */
+  int32_t inlet_bc;    /* hlb_iolet_bc used for sites of type INLET */
+  int32_t outlet_bc;   /* hlb_iolet_bc used for sites of type OUTLET */
+  int32_t storage;     /* hlb_storage */
+  double tau;          /* LbmParameters::GetTau() (LbmParameters.h:158-163) */
+  double rho0;         /* reference density for FP32/FP16 storage (ignored
+                          for FP64) */
+  int32_t device;      /* HIP device ordinal */
+  int32_t reserved;
+} hlb_params;
+
+typedef struct {
+  int64_t n_sites;
+  const int32_t* coords;       /* [n][3] global site coordinates            */
+  const uint32_t* site_type;   /* [n] hlb_site_type                         */
+  const uint32_t* wall_mask;   /* [n] bit d-1: link d crosses a wall        */
+  const uint32_t* iolet_mask;  /* [n] bit d-1: link d crosses an iolet      */
+  const int32_t* iolet_id;     /* [n] index into inlets (INLET sites) or
+                                  outlets (OUTLET sites); ignored otherwise */
+  const float* cut_distance;   /* [n][Q-1] cut distance of link d (as read
+                                  from the .gmy file); required for BFL     */
+  const float* wall_normal;    /* [n][3] optional (wall shear stress)       */
+} hlb_domain;
+
+typedef struct {
+  int32_t kind;                /* hlb_iolet_kind */
+  int32_t reserved;
+  double position[3];          /* lattice units (InOutLet::SetPosition) */
+  double normal[3];            /* normalised by the library as
+                                  InOutLet::SetNormal does */
+  /* cosine pressure: density = mean + amp*cos(2*pi*t/period + phase),
+   * linear warm-up from the minimum simulation density over `warmup`
+   * steps (InOutLetCosine.cc:26-43) */
+  double density_mean, density_amp, phase, period;
+  int64_t warmup;
+  /* velocity iolets */
+  double radius;               /* lattice units */
+  double max_speed;            /* parabolic: centreline speed */
+  double pressure_gradient_amplitude;  /* Womersley (period shared) */
+  double womersley_number;
+} hlb_iolet;
+
+/* Reductions over the current state (the reference's fOld), all binary64:
+ * mass = sum of density; min/max density; max |u|; `stable` is 0 if any
+ * distribution is not > 0 (StabilityTester.h:91-118 criterion);
+ * iolet_flux[k] = sum over the sites touching iolet k of (momentum . normal),
+ * inlets first then outlets. */
+typedef struct {
+  double mass;
+  double density_min, density_max;
+  double velocity_max;
+  int32_t stable;
+  int32_t n_iolets;
+  double iolet_flux[64];
+} hlb_monitor;
+
+int hlb_abi_version(void);
+const char* hlb_last_error(void);
+
+/* Build a solver for one rank's domain.  The iolet arrays are the inlet and
+ * outlet BoundaryValues (Code/lb/iolets/BoundaryValues.cc:13-58); the
+ * minimum simulation density passed to each iolet is computed as
+ * LBM::PrepareBoundaryObjects does (Code/lb/lb.hpp:127-152).  The state is
+ * undefined until hlb_set_equilibrium or hlb_set_distributions. */
+int hlb_create(const hlb_params* params, const hlb_domain* domain,
+               const hlb_iolet* inlets, int32_t n_inlets,
+               const hlb_iolet* outlets, int32_t n_outlets,
+               hlb_solver** out);
+void hlb_destroy(hlb_solver* s);
+
+/* EquilibriumInitialCondition::SetFs (Code/lb/InitialCondition.hpp:39-52):
+ * every site gets feq(density, momentum). Resets the time to 1. */
+int hlb_set_equilibrium(hlb_solver* s, double density, const double momentum[3]);
+/* SimulationState time step (1-indexed, SimulationState.cc:15-49). */
+int hlb_set_time(hlb_solver* s, uint64_t time_step);
+uint64_t hlb_get_time(const hlb_solver* s);
+
+/* Advance n time steps (n x SimulationMaster::DoTimeStep for the LB actor:
+ * stream-and-collide, PostStep, swap). */
+int hlb_step(hlb_solver* s, int64_t n);
+
+/* fOld in the caller's order, double[n][Q] (checkpoint export/import,
+ * Code/extraction/LbDataSourceIterator.cc:76-80,
+ * Code/extraction/LocalDistributionInput.cc:38-80). */
+int hlb_get_distributions(hlb_solver* s, double* f_out);
+int hlb_set_distributions(hlb_solver* s, const double* f_in);
+
+/* Density [n] and velocity [n][3] of fOld in the caller's order (either may
+ * be NULL), as MacroscopicPropertyCache / LbDataSourceIterator compute them. */
+int hlb_get_macroscopic(hlb_solver* s, double* density, double* velocity);
+
+int hlb_reduce(hlb_solver* s, hlb_monitor* out);
+
+/* Convergence monitor: hlb_snapshot_velocity stores the current velocity
+ * field on the device; hlb_velocity_change returns
+ * max over sites of |u - u_snapshot| / reference_speed
+ * (StabilityTester::ComputeRelativeDifference, StabilityTester.h:160-183). */
+int hlb_snapshot_velocity(hlb_solver* s);
+int hlb_velocity_change(hlb_solver* s, double reference_speed, double* out);
+
+/* Halo exchange for a subdomain (Code/geometry/FieldData.cc:27-49): pack the
+ * post-collision distributions that leave through the given (site, direction)
+ * pairs into a device buffer, or unpack received values into the ghost
+ * slots.  `pairs` is int64[count][2] of (caller site index, direction). */
+int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count,
+                  double* host_buffer);
+
+/* Device timing of n steps with HIP events on the solver's stream
+ * (milliseconds, launch gaps included). */
+int hlb_time_steps(hlb_solver* s, int64_t n, double* elapsed_ms);
+
+/* Introspection. */
+int64_t hlb_site_count(const hlb_solver* s);
+int32_t hlb_lattice_q(const hlb_solver* s);
+/* Bytes of device memory held by the solver. */
+int64_t hlb_device_bytes(const hlb_solver* s);
+/* The HIP stream (hipStream_t) the solver launches on. */
+void* hlb_stream(const hlb_solver* s);
+int hlb_synchronize(hlb_solver* s);
+
+/* Stand-alone helpers mirroring the reference's parameter set-up. */
+/* LbmParameters::CalcDerivedParams (LbmParameters.h:158-163). */
+double hlb_tau_from_physical(double time_step_s, double voxel_size_m,
+                             double fluid_density_kgm3, double viscosity_pas);
+/* util::UnitConverter (Code/util/UnitConverter.cc:14-48). */
+double hlb_pressure_to_lattice(double p_mmhg, double reference_mmhg,
+                               double time_step_s, double voxel_size_m,
+                               double fluid_density_kgm3);
+double hlb_pressure_difference_to_lattice(double dp_mmhg, double time_step_s,
+                                          double voxel_size_m,
+                                          double fluid_density_kgm3);
+double hlb_pressure_gradient_to_lattice(double pg_mmhg_per_m,
+                                        double time_step_s,
+                                        double voxel_size_m,
+                                        double fluid_density_kgm3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
