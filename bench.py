@@ -95,6 +95,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--storage", choices=sorted(STORE_BYTES), default="fp64")
+    ap.add_argument("--layout", choices=["blocks", "pull_table"], default="blocks")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -108,7 +109,7 @@ def main() -> int:
     dom, traits, tau, inlets, outlets, extra = tree_config(args.storage)
     t0 = time.time()
     lbm = H.LBM(traits, dom, H.LbmParameters(tau=tau), inlets, outlets,
-                storage=args.storage, rho0=1.0)
+                storage=args.storage, rho0=1.0, layout=args.layout)
     lbm.SetInitialConditions(1.0)
     lbm.synchronize()
     setup_s = time.time() - t0
@@ -155,7 +156,8 @@ def main() -> int:
         "config": {"workload": "vessel_tree_5gen_R40_D3Q19_LBGK_BFL_LaddWomersley_Nash",
                    "fluid_sites": n, "lattice": "D3Q19", "kernel": "LBGK", "tau": tau,
                    "walls": "BFL", "inlet": "Ladd/Womersley", "outlets": len(outlets),
-                   "storage": args.storage, "setup_s": round(setup_s, 2),
+                   "storage": args.storage, "layout": args.layout,
+                   "setup_s": round(setup_s, 2),
                    "bytes_alg_per_site": b_alg, **{k: round(v, 6) for k, v in extra.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -20,6 +20,7 @@ HLB_IOLET_NASH, HLB_IOLET_LADD = 0, 1
 HLB_IOLET_COSINE, HLB_IOLET_PARABOLIC, HLB_IOLET_WOMERSLEY = 0, 1, 2
 HLB_SITE_FLUID, HLB_SITE_INLET, HLB_SITE_OUTLET = 1, 2, 3
 HLB_STORE_FP64, HLB_STORE_FP32, HLB_STORE_FP16 = 0, 1, 2
+HLB_LAYOUT_BLOCKS, HLB_LAYOUT_PULL_TABLE = 0, 1
 HLBG_CUT_NONE, HLBG_CUT_WALL, HLBG_CUT_INLET, HLBG_CUT_OUTLET = 0, 1, 2, 3
 
 
@@ -27,7 +28,7 @@ class hlb_params(C.Structure):
     _fields_ = [("lattice", C.c_int32), ("kernel", C.c_int32), ("wall", C.c_int32),
                 ("inlet_bc", C.c_int32), ("outlet_bc", C.c_int32),
                 ("storage", C.c_int32), ("tau", C.c_double), ("rho0", C.c_double),
-                ("device", C.c_int32), ("reserved", C.c_int32)]
+                ("device", C.c_int32), ("layout", C.c_int32)]
 
 
 class hlb_domain(C.Structure):

@@ -12,11 +12,22 @@ constexpr int kMaxMoments = 15;
 constexpr int kMaxQ = 19;
 
 // Device-side description of one solver's domain in the library's own
-// layout: sites sorted into 8^3 blocks in Morton order, z-fastest inside a
-// block, distributions stored structure-of-arrays with stride `np`.
+// layout.  Sites are ordered by 8^3 block in Morton order, z-fastest inside a
+// block ("compact" order).  Distributions are structure-of-arrays with stride
+// `np`, indexed either by compact site (layout 0, with a 32-bit pull table)
+// or by slot in dense 512-slot blocks (layout 1, neighbours by arithmetic).
 struct DevDomain {
+  int layout = 1;   // 0 = compact + pull table, 1 = dense blocks
   int64_t n = 0;    // fluid sites
-  int64_t np = 0;   // padded stride of every per-direction array
+  int64_t np = 0;   // stride of every per-direction state array
+  int64_t npc = 0;  // stride of compact binary64 exports (n padded)
+  // Block layout: 8 waves (x planes) per block.
+  uint32_t nwaves = 0;
+  const int32_t* bnbr = nullptr;       // [blocks][27] neighbour block or -1
+  const uint64_t* wave_fluid = nullptr;  // [nwaves] lane is a fluid site
+  const uint64_t* wave_bnd = nullptr;    // [nwaves] lane has a boundary record
+  const uint32_t* wave_rec = nullptr;    // [nwaves] first boundary record
+  const uint32_t* wave_cbase = nullptr;  // [nwaves] compact index of first fluid lane
   // Pull table: src[(d-1)*np + x] is the site whose post-collision value of
   // direction d streams into slot d of site x (the site at x - c_d).  For a
   // slot fed by a wall or iolet link it holds x itself (never dereferenced
@@ -49,9 +60,8 @@ struct StepArgs {
   double omega = 0;   // -1/tau
   double rho0 = 1;    // shift for reduced-precision storage
   float omega_f = 0;
-  // MRT: device arrays of (S_k * M[k][d] / diag_k), index k*Q + d
-  const double* mrt_c = nullptr;
-  const float* mrt_c_f = nullptr;
+  double mrt_s_tau = 0;  // MRT: the relaxation rate 1/tau
+
   // Boundary values of the step being streamed: per iolet, two doubles
   // (Nash: ghost density; Ladd Womersley: e^{i w t}; Ladd parabolic: speed).
   const double* bv = nullptr;
@@ -60,23 +70,22 @@ struct StepArgs {
 
 // Launches; every function returns 0 or a HIP error code.
 // Every site gets the post-stream distributions vals[Q] (binary64).
-int launch_fill(const StepArgs& a, int64_t n, int64_t np, const double* vals,
+int launch_fill(const DevDomain& d, const StepArgs& a, const double* vals,
                 void* state, void* stream);
 int launch_step(const DevDomain& d, const StepArgs& a, const void* fin,
                 void* fout, void* stream);
 int launch_collide_only(const DevDomain& d, const StepArgs& a, const void* fin,
                         void* fout, void* stream);
-// Post-stream distributions (the reference's fOld) as binary64 SoA [Q][np].
-// `from_post_stream` copies fin when the state already holds them.
+// Post-stream distributions (the reference's fOld) as binary64 compact SoA
+// [Q][npc].  `from_post_stream` copies fin when the state already holds them.
 int launch_stream_only(const DevDomain& d, const StepArgs& a, const void* fin,
                        double* s_out, int from_post_stream, void* stream);
-// SoA [Q][np] binary64 <-> caller-order AoS [n][Q] through perm
-// (perm[internal] = caller index).
+// Compact SoA [Q][np] binary64 <-> caller-order AoS [n][Q] through perm
+// (perm[compact] = caller index).
 int launch_soa_to_aos(int Q, int64_t n, int64_t np, const double* soa,
                       const int64_t* perm, double* aos, void* stream);
-int launch_aos_to_state(const StepArgs& a, int64_t n, int64_t np,
-                        const double* aos, const int64_t* perm, void* state,
-                        void* stream);
+int launch_import(const DevDomain& d, const StepArgs& a, const double* aos,
+                  const int64_t* perm, void* state, void* stream);
 // Moments of a binary64 SoA state: rho[np], u[3][np] (internal order).
 int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
                    double* u, void* stream);

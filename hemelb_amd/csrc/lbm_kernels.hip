@@ -229,6 +229,22 @@ struct Hydro {
   T rho, mx, my, mz;  // rho is the full density (rho0 added back if shifted)
 };
 
+// d'Humieres relaxation rates (DHumieresD3Q15MRTBasis.cc:10-26,
+// DHumieresD3Q19MRTBasis.cc:11-36); 0 marks the moments relaxed with 1/tau.
+__host__ __device__ constexpr double mrt_rate(int Q, int k) {
+  constexpr double r15[11] = {1.6, 1.2, 1.6, 1.6, 1.6, 0, 0, 0, 0, 0, 1.2};
+  constexpr double r19[15] = {1.19, 1.4, 1.2, 1.2, 1.2, 0, 1.4, 0,
+                              1.4,  0,   0,   0,   1.98, 1.98, 1.98};
+  return Q == 15 ? r15[k] : r19[k];
+}
+// BASIS_TIMES_BASIS_TRANSPOSED (basis_helpers.h:14-26), exact in binary64.
+template <int Q>
+__host__ __device__ constexpr double mrt_diag(int k) {
+  double acc = 0.0;
+  for (int i = 0; i < Q; ++i) acc = acc + (double)Lat<Q>::M[k][i] * (double)Lat<Q>::M[k][i];
+  return acc;
+}
+
 // Collide in place: f (post-stream, possibly shifted) -> f* ; returns the
 // moments the streamers of the next step need.
 template <int Q, int KERNEL, bool SHIFT, typename T>
@@ -263,6 +279,9 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
     });
   } else {
     // MRT: m_neq = M f_neq (MRT.h:123-135); f* = f - sum_k (S_k N_kd) m_neq_k
+    // with N = M / diag(M M^T) (MRT.h:139-152).  S_k N_kd is a compile-time
+    // binary64 product except for the 1/tau moments, where it is formed from
+    // the runtime 1/tau exactly as the reference forms it.
     constexpr int NM = L::nmom;
     T mneq[NM];
     sfor<0, NM>([&](auto ki) {
@@ -277,20 +296,116 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
       });
       mneq[k] = acc;
     });
-    const T* __restrict__ C;
-    if constexpr (std::is_same<T, double>::value) C = a.mrt_c; else C = a.mrt_c_f;
+    const double s_tau = a.mrt_s_tau;
     sfor<0, Q>([&](auto di) {
       constexpr int d = decltype(di)::value;
       T coll = T(0.);
       sfor<0, NM>([&](auto ki) {
         constexpr int k = decltype(ki)::value;
-        if constexpr (L::M[k][d] != 0) coll += C[k * Q + d] * mneq[k];
+        if constexpr (L::M[k][d] != 0) {
+          constexpr double N = (double)L::M[k][d] / mrt_diag<Q>(k);
+          constexpr double rate = mrt_rate(Q, k);
+          double c;
+          if constexpr (rate != 0.0) {
+            constexpr double cc = rate * N;
+            c = cc;
+          } else {
+            c = s_tau * N;
+          }
+          coll += T(c) * mneq[k];
+        }
       });
       f[d] = f[d] - coll;
     });
   }
   return h;
 }
+
+// ---------------------------------------------------------------------------
+// Layouts: where a thread's site lives and where its neighbours are.
+// ---------------------------------------------------------------------------
+struct SiteRef {
+  int64_t x;      // index into every per-direction array
+  uint32_t rec;   // boundary record (valid when boundary)
+  bool boundary;
+  uint32_t w;     // block layout: wave index (block * 8 + x plane)
+  uint32_t lane;  // block layout: lane = y * 8 + z inside the plane
+};
+
+__device__ __forceinline__ uint32_t logical_workgroup() {
+  // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give
+  // every XCD one contiguous range of logical workgroups.
+  const uint32_t nwg = gridDim.x, b = blockIdx.x;
+  const uint32_t xcd = b & 7u, k = b >> 3;
+  const uint32_t base = nwg >> 3, rem = nwg & 7u;
+  const uint32_t start = xcd * base + (xcd < rem ? xcd : rem);
+  return start + k;
+}
+
+// Compact layout: sites packed in Morton-block order, 32-bit pull table.
+struct SparseLayout {
+  static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
+    r.x = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
+    if (r.x >= D.n) return false;
+    const uint32_t bi = D.binfo[r.x];
+    r.boundary = bi != 0;
+    r.rec = bi - 1;
+    return true;
+  }
+  template <int Q, int E>
+  static __device__ __forceinline__ int64_t src(const DevDomain& D, const SiteRef& r) {
+    return D.src[(int64_t)(E - 1) * D.np + r.x];
+  }
+  static __device__ __forceinline__ int64_t compact(const DevDomain&, const SiteRef& r) {
+    return r.x;
+  }
+};
+
+// Block layout: every 8^3 block holding fluid owns 512 slots (x-plane major,
+// z fastest); a wave covers one x plane of one block.  Neighbours come from
+// slot arithmetic and the block's 27-entry neighbour table; per-wave 64-bit
+// masks say which lanes are fluid and which carry a boundary record.
+struct BlockLayout {
+  static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
+    const int64_t slot = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
+    r.w = __builtin_amdgcn_readfirstlane((uint32_t)(slot >> 6));
+    r.lane = threadIdx.x & 63u;
+    r.x = ((int64_t)r.w << 6) + r.lane;
+    if (r.w >= D.nwaves) return false;
+    const uint64_t fm = D.wave_fluid[r.w];
+    if (!((fm >> r.lane) & 1ull)) return false;
+    const uint64_t bm = D.wave_bnd[r.w];
+    r.boundary = (bm >> r.lane) & 1ull;
+    r.rec = D.wave_rec[r.w] + (uint32_t)__popcll(bm & ((1ull << r.lane) - 1ull));
+    return true;
+  }
+  template <int Q, int E>
+  static __device__ __forceinline__ int64_t src(const DevDomain& D, const SiteRef& r) {
+    constexpr int CX = Lat<Q>::c[E][0], CY = Lat<Q>::c[E][1], CZ = Lat<Q>::c[E][2];
+    const int lx = (int)(r.w & 7u);
+    const int ly = (int)(r.lane >> 3), lz = (int)(r.lane & 7u);
+    // source = x - c_E; the wave-uniform x shift may leave the block
+    const int sx = lx - CX;
+    const int ox = sx < 0 ? -1 : (sx > 7 ? 1 : 0);
+    const bool cy = CY != 0 && (CY > 0 ? ly == 0 : ly == 7);
+    const bool cz = CZ != 0 && (CZ > 0 ? lz == 0 : lz == 7);
+    const int32_t* nbt = D.bnbr + (int64_t)(r.w >> 3) * 27;
+    const int base = 13 + 9 * ox;
+    const int32_t t00 = nbt[base];
+    int32_t nb = t00;
+    if constexpr (CY != 0 || CZ != 0) {
+      const int32_t t10 = nbt[base - 3 * CY];
+      const int32_t t01 = nbt[base - CZ];
+      const int32_t t11 = nbt[base - 3 * CY - CZ];
+      nb = cy ? (cz ? t11 : t10) : (cz ? t01 : t00);
+    }
+    return ((int64_t)nb << 9) + (((sx & 7) << 6) | (((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
+  }
+  static __device__ __forceinline__ int64_t compact(const DevDomain& D, const SiteRef& r) {
+    const uint64_t fm = D.wave_fluid[r.w];
+    return (int64_t)D.wave_cbase[r.w] + __popcll(fm & ((1ull << r.lane) - 1ull));
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Streaming (pull) with boundary links
@@ -343,25 +458,23 @@ __device__ __forceinline__ double ladd_correction(const DevDomain& D,
   return 2. * L::w[DP] * dot / kCs2;
 }
 
-// Post-stream distributions of site x (the reference's fNew after
+// Post-stream distributions of a site (the reference's fNew after
 // StreamAndCollide and PostStep) from the post-collision state `fin`.
-template <int Q, typename S, typename T>
+template <int Q, typename S, typename T, class Lay>
 __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
-                                       const S* __restrict__ fin, int64_t x,
+                                       const S* __restrict__ fin, const SiteRef& r,
                                        T (&f)[Q]) {
   using St = Store<S>;
-  const int64_t np = D.np;
-  const uint32_t bi = D.binfo[x];
+  const int64_t np = D.np, x = r.x;
   f[0] = St::ld(fin + x);
-  if (bi == 0) {
+  if (!r.boundary) {
     sfor<1, Q>([&](auto ei) {
       constexpr int e = decltype(ei)::value;
-      const uint32_t s = D.src[(int64_t)(e - 1) * np + x];
-      f[e] = St::ld(fin + (int64_t)e * np + s);
+      f[e] = St::ld(fin + (int64_t)e * np + Lay::template src<Q, e>(D, r));
     });
     return;
   }
-  const uint32_t rec = bi - 1;
+  const uint32_t rec = r.rec;
   const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
   const int io = D.b_io[rec];
   const int k = io >= 0 ? D.io_iolet[io] : 0;
@@ -394,8 +507,7 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
         }
       }
     } else {
-      const uint32_t s = D.src[(int64_t)(e - 1) * np + x];
-      f[e] = St::ld(fin + (int64_t)e * np + s);
+      f[e] = St::ld(fin + (int64_t)e * np + Lay::template src<Q, e>(D, r));
     }
   });
   if (A.wall == 1 && wm) {
@@ -412,23 +524,11 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
   }
 }
 
-__device__ __forceinline__ int64_t site_index() {
-  // XCD-aware remap: hardware dispatches workgroup b to XCD b % 8; give
-  // every XCD one contiguous range of logical workgroups.
-  const uint32_t nwg = gridDim.x, b = blockIdx.x;
-  const uint32_t xcd = b & 7u, k = b >> 3;
-  const uint32_t base = nwg >> 3, rem = nwg & 7u;
-  const uint32_t start = xcd * base + (xcd < rem ? xcd : rem);
-  const uint32_t logical = start + k;
-  return (int64_t)logical * blockDim.x + threadIdx.x;
-}
-
 template <int Q, typename T>
-__device__ __forceinline__ void update_stash(const DevDomain& D, int64_t x,
+__device__ __forceinline__ void update_stash(const DevDomain& D, const SiteRef& r,
                                              const Hydro<Q, T>& h) {
-  const uint32_t bi = D.binfo[x];
-  if (bi == 0) return;
-  const int io = D.b_io[bi - 1];
+  if (!r.boundary) return;
+  const int io = D.b_io[r.rec];
   if (io < 0) return;
   const double rho = (double)h.rho;
   if (D.io_bc[io] == 0) {
@@ -444,15 +544,22 @@ __device__ __forceinline__ void update_stash(const DevDomain& D, int64_t x,
   }
 }
 
-template <int Q, int KERNEL, typename S>
+template <typename S>
+__device__ __forceinline__ void store_out(S* p, typename Store<S>::T v) {
+  // Written once per step and read only by the next launch: stream it past
+  // the caches.
+  __builtin_nontemporal_store((S)v, p);
+}
+
+template <int Q, int KERNEL, typename S, class Lay>
 __global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
                                               const S* __restrict__ fin,
                                               S* __restrict__ fout) {
   using T = typename Store<S>::T;
-  const int64_t x = site_index();
-  if (x >= D.n) return;
+  SiteRef r;
+  if (!Lay::locate(D, r)) return;
   T f[Q];
-  gather<Q, S, T>(D, A, fin, x, f);
+  gather<Q, S, T, Lay>(D, A, fin, r, f);
   bool ok = true;
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
@@ -463,55 +570,86 @@ __global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
   });
   if (!ok) atomicOr(A.unstable, 1);
   const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, A);
-  update_stash<Q, T>(D, x, h);
+  update_stash<Q, T>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    Store<S>::st(fout + (int64_t)d * D.np + x, f[d]);
+    store_out<S>(fout + (int64_t)d * D.np + r.x, f[d]);
   });
 }
 
-template <int Q, int KERNEL, typename S>
+template <int Q, int KERNEL, typename S, class Lay>
 __global__ __launch_bounds__(256) void k_collide_only(DevDomain D, StepArgs A,
                                                       const S* __restrict__ fin,
                                                       S* __restrict__ fout) {
   using T = typename Store<S>::T;
-  const int64_t x = site_index();
-  if (x >= D.n) return;
+  SiteRef r;
+  if (!Lay::locate(D, r)) return;
   T f[Q];
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    f[d] = Store<S>::ld(fin + (int64_t)d * D.np + x);
+    f[d] = Store<S>::ld(fin + (int64_t)d * D.np + r.x);
   });
   const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, A);
-  update_stash<Q, T>(D, x, h);
+  update_stash<Q, T>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    Store<S>::st(fout + (int64_t)d * D.np + x, f[d]);
+    Store<S>::st(fout + (int64_t)d * D.np + r.x, f[d]);
   });
 }
 
-template <int Q, typename S>
+// Post-stream distributions in binary64, compact SoA [Q][npc].
+template <int Q, typename S, class Lay>
 __global__ __launch_bounds__(256) void k_stream_only(DevDomain D, StepArgs A,
                                                      const S* __restrict__ fin,
                                                      double* __restrict__ out,
                                                      int from_post_stream) {
   using T = typename Store<S>::T;
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= D.n) return;
+  SiteRef r;
+  if (!Lay::locate(D, r)) return;
   T f[Q];
   if (from_post_stream) {
     sfor<0, Q>([&](auto di) {
       constexpr int d = decltype(di)::value;
-      f[d] = Store<S>::ld(fin + (int64_t)d * D.np + x);
+      f[d] = Store<S>::ld(fin + (int64_t)d * D.np + r.x);
     });
   } else {
-    gather<Q, S, T>(D, A, fin, x, f);
+    gather<Q, S, T, Lay>(D, A, fin, r, f);
   }
+  const int64_t c = Lay::compact(D, r);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
     double v = (double)f[d];
     if constexpr (kShifted<S>) v += Lat<Q>::w[d] * A.rho0;
-    out[(int64_t)d * D.np + x] = v;
+    out[(int64_t)d * D.npc + c] = v;
+  });
+}
+
+// Caller-order AoS [n][Q] -> state, perm[compact] = caller index.
+template <int Q, typename S, class Lay>
+__global__ __launch_bounds__(256) void k_import(DevDomain D, const double* aos,
+                                                const int64_t* perm, S* state,
+                                                double rho0) {
+  SiteRef r;
+  if (!Lay::locate(D, r)) return;
+  const int64_t c = perm[Lay::compact(D, r)];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    double v = aos[c * Q + d];
+    if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
+    Store<S>::st(state + (int64_t)d * D.np + r.x, (typename Store<S>::T)v);
+  });
+}
+
+template <int Q, typename S>
+__global__ void k_fill(int64_t nslots, int64_t np, const double* vals, S* state,
+                       double rho0) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nslots) return;
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    double v = vals[d];
+    if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
+    Store<S>::st(state + (int64_t)d * np + x, (typename Store<S>::T)v);
   });
 }
 
@@ -521,29 +659,6 @@ __global__ void k_soa_to_aos(int Q, int64_t n, int64_t np, const double* soa,
   if (x >= n) return;
   const int64_t c = perm[x];
   for (int d = 0; d < Q; ++d) aos[c * Q + d] = soa[(int64_t)d * np + x];
-}
-
-__global__ void k_aos_to_state(int Q, int64_t n, int64_t np, const double* aos,
-                               const int64_t* perm, double* state) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  const int64_t c = perm[x];
-  for (int d = 0; d < Q; ++d) state[(int64_t)d * np + x] = aos[c * Q + d];
-}
-
-template <int Q>
-__global__ void k_aos_to_shifted(int64_t n, int64_t np, const double* aos,
-                                 const int64_t* perm, float* s32, _Float16* s16,
-                                 double rho0) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  const int64_t c = perm[x];
-  sfor<0, Q>([&](auto di) {
-    constexpr int d = decltype(di)::value;
-    const double v = aos[c * Q + d] - Lat<Q>::w[d] * rho0;
-    if (s32) s32[(int64_t)d * np + x] = (float)v;
-    else s16[(int64_t)d * np + x] = (_Float16)(float)v;
-  });
 }
 
 template <int Q>
@@ -642,91 +757,92 @@ __global__ void k_gather_pairs(int64_t count, int64_t np, const int64_t* slot,
   out[i] = v;
 }
 
-template <int Q, typename S>
-__global__ void k_fill(int64_t n, int64_t np, const double* vals, S* state,
-                       double rho0) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  sfor<0, Q>([&](auto di) {
-    constexpr int d = decltype(di)::value;
-    double v = vals[d];
-    if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
-    Store<S>::st(state + (int64_t)d * np + x, (typename Store<S>::T)v);
-  });
-}
-
 inline dim3 grid_for(int64_t n, int threads = 256) {
   return dim3((unsigned)((n + threads - 1) / threads));
 }
 
-template <int Q, int KERNEL, typename S>
-int step_t(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
-           hipStream_t st) {
-  hipLaunchKernelGGL((k_step<Q, KERNEL, S>), grid_for(d.n), dim3(256), 0, st, d,
-                     a, (const S*)fin, (S*)fout);
-  return (int)hipGetLastError();
+// Threads launched for a per-site kernel: compact sites or all slots.
+inline int64_t thread_count(const DevDomain& d) {
+  return d.layout == 0 ? d.n : (int64_t)d.nwaves * 64;
 }
 
-template <int Q, int KERNEL, typename S>
-int collide_t(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
-              hipStream_t st) {
-  hipLaunchKernelGGL((k_collide_only<Q, KERNEL, S>), grid_for(d.n), dim3(256), 0,
-                     st, d, a, (const S*)fin, (S*)fout);
-  return (int)hipGetLastError();
-}
-
+// (layout, lattice, kernel, storage) -> instantiation
 template <typename Fn>
-int dispatch(const StepArgs& a, Fn&& fn) {
-  // (lattice, kernel, storage) -> instantiation
-#define HLB_CASE(Q_, K_, S_) \
-  if (a.Q == Q_ && a.kernel == K_ && a.storage == (std::is_same<S_, double>::value ? 0 : std::is_same<S_, float>::value ? 1 : 2)) \
-    return fn(std::integral_constant<int, Q_>{}, std::integral_constant<int, K_>{}, (S_*)nullptr);
-  HLB_CASE(15, 0, double) HLB_CASE(15, 1, double) HLB_CASE(19, 0, double) HLB_CASE(19, 1, double)
-  HLB_CASE(15, 0, float) HLB_CASE(15, 1, float) HLB_CASE(19, 0, float) HLB_CASE(19, 1, float)
-  HLB_CASE(15, 0, _Float16) HLB_CASE(15, 1, _Float16) HLB_CASE(19, 0, _Float16) HLB_CASE(19, 1, _Float16)
+int dispatch(const DevDomain& d, const StepArgs& a, Fn&& fn) {
+#define HLB_CASE(L_, Q_, K_, S_, SC_)                                        \
+  if (d.layout == L_ && a.Q == Q_ && a.kernel == K_ && a.storage == SC_)    \
+    return fn(std::integral_constant<int, Q_>{}, std::integral_constant<int, K_>{}, \
+              (S_*)nullptr, (std::conditional_t<L_ == 0, SparseLayout, BlockLayout>*)nullptr);
+#define HLB_CASES(L_)                                                        \
+  HLB_CASE(L_, 15, 0, double, 0) HLB_CASE(L_, 15, 1, double, 0)             \
+  HLB_CASE(L_, 19, 0, double, 0) HLB_CASE(L_, 19, 1, double, 0)             \
+  HLB_CASE(L_, 15, 0, float, 1) HLB_CASE(L_, 15, 1, float, 1)               \
+  HLB_CASE(L_, 19, 0, float, 1) HLB_CASE(L_, 19, 1, float, 1)               \
+  HLB_CASE(L_, 15, 0, _Float16, 2) HLB_CASE(L_, 15, 1, _Float16, 2)         \
+  HLB_CASE(L_, 19, 0, _Float16, 2) HLB_CASE(L_, 19, 1, _Float16, 2)
+  HLB_CASES(0)
+  HLB_CASES(1)
+#undef HLB_CASES
 #undef HLB_CASE
   return (int)hipErrorInvalidValue;
 }
 
 }  // namespace
 
-int launch_fill(const StepArgs& a, int64_t n, int64_t np, const double* vals,
-                void* state, void* stream) {
-  return dispatch(a, [&](auto q, auto, auto* s) {
+int launch_fill(const DevDomain& d, const StepArgs& a, const double* vals, void* state,
+                void* stream) {
+  return dispatch(d, a, [&](auto q, auto, auto* s, auto*) {
     using S = std::remove_pointer_t<decltype(s)>;
     constexpr int Q = decltype(q)::value;
-    hipLaunchKernelGGL((k_fill<Q, S>), grid_for(n), dim3(256), 0,
-                       (hipStream_t)stream, n, np, vals, (S*)state, a.rho0);
+    hipLaunchKernelGGL((k_fill<Q, S>), grid_for(d.np), dim3(256), 0, (hipStream_t)stream,
+                       d.np, d.np, vals, (S*)state, a.rho0);
     return (int)hipGetLastError();
   });
 }
 
 int launch_step(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
                 void* stream) {
-  return dispatch(a, [&](auto q, auto k, auto* s) {
+  return dispatch(d, a, [&](auto q, auto k, auto* s, auto* lay) {
     using S = std::remove_pointer_t<decltype(s)>;
-    return step_t<decltype(q)::value, decltype(k)::value, S>(d, a, fin, fout,
-                                                             (hipStream_t)stream);
+    using Lay = std::remove_pointer_t<decltype(lay)>;
+    hipLaunchKernelGGL((k_step<decltype(q)::value, decltype(k)::value, S, Lay>),
+                       grid_for(thread_count(d)), dim3(256), 0, (hipStream_t)stream, d, a,
+                       (const S*)fin, (S*)fout);
+    return (int)hipGetLastError();
   });
 }
 
 int launch_collide_only(const DevDomain& d, const StepArgs& a, const void* fin,
                         void* fout, void* stream) {
-  return dispatch(a, [&](auto q, auto k, auto* s) {
+  return dispatch(d, a, [&](auto q, auto k, auto* s, auto* lay) {
     using S = std::remove_pointer_t<decltype(s)>;
-    return collide_t<decltype(q)::value, decltype(k)::value, S>(
-        d, a, fin, fout, (hipStream_t)stream);
+    using Lay = std::remove_pointer_t<decltype(lay)>;
+    hipLaunchKernelGGL((k_collide_only<decltype(q)::value, decltype(k)::value, S, Lay>),
+                       grid_for(thread_count(d)), dim3(256), 0, (hipStream_t)stream, d, a,
+                       (const S*)fin, (S*)fout);
+    return (int)hipGetLastError();
   });
 }
 
 int launch_stream_only(const DevDomain& d, const StepArgs& a, const void* fin,
                        double* s_out, int from_post_stream, void* stream) {
-  return dispatch(a, [&](auto q, auto, auto* s) {
+  return dispatch(d, a, [&](auto q, auto, auto* s, auto* lay) {
     using S = std::remove_pointer_t<decltype(s)>;
-    constexpr int Q = decltype(q)::value;
-    hipLaunchKernelGGL((k_stream_only<Q, S>), grid_for(d.n), dim3(256), 0,
-                       (hipStream_t)stream, d, a, (const S*)fin, s_out,
+    using Lay = std::remove_pointer_t<decltype(lay)>;
+    hipLaunchKernelGGL((k_stream_only<decltype(q)::value, S, Lay>), grid_for(thread_count(d)),
+                       dim3(256), 0, (hipStream_t)stream, d, a, (const S*)fin, s_out,
                        from_post_stream);
+    return (int)hipGetLastError();
+  });
+}
+
+int launch_import(const DevDomain& d, const StepArgs& a, const double* aos,
+                  const int64_t* perm, void* state, void* stream) {
+  return dispatch(d, a, [&](auto q, auto, auto* s, auto* lay) {
+    using S = std::remove_pointer_t<decltype(s)>;
+    using Lay = std::remove_pointer_t<decltype(lay)>;
+    hipLaunchKernelGGL((k_import<decltype(q)::value, S, Lay>), grid_for(thread_count(d)),
+                       dim3(256), 0, (hipStream_t)stream, d, aos, perm, (S*)state, a.rho0);
     return (int)hipGetLastError();
   });
 }
@@ -735,26 +851,6 @@ int launch_soa_to_aos(int Q, int64_t n, int64_t np, const double* soa,
                       const int64_t* perm, double* aos, void* stream) {
   hipLaunchKernelGGL(k_soa_to_aos, grid_for(n), dim3(256), 0, (hipStream_t)stream,
                      Q, n, np, soa, perm, aos);
-  return (int)hipGetLastError();
-}
-
-int launch_aos_to_state(const StepArgs& a, int64_t n, int64_t np,
-                        const double* aos, const int64_t* perm, void* state,
-                        void* stream) {
-  if (a.storage == 0) {
-    hipLaunchKernelGGL(k_aos_to_state, grid_for(n), dim3(256), 0,
-                       (hipStream_t)stream, a.Q, n, np, aos, perm,
-                       (double*)state);
-    return (int)hipGetLastError();
-  }
-  float* s32 = a.storage == 1 ? (float*)state : nullptr;
-  _Float16* s16 = a.storage == 2 ? (_Float16*)state : nullptr;
-  if (a.Q == 15)
-    hipLaunchKernelGGL(k_aos_to_shifted<15>, grid_for(n), dim3(256), 0,
-                       (hipStream_t)stream, n, np, aos, perm, s32, s16, a.rho0);
-  else
-    hipLaunchKernelGGL(k_aos_to_shifted<19>, grid_for(n), dim3(256), 0,
-                       (hipStream_t)stream, n, np, aos, perm, s32, s16, a.rho0);
   return (int)hipGetLastError();
 }
 

@@ -219,7 +219,7 @@ size_t storage_bytes(int storage) { return storage == 0 ? 8 : storage == 1 ? 4 :
 struct hlb_solver {
   hlb_params p{};
   int Q = 19;
-  int64_t n = 0, np = 0;
+  int64_t n = 0, np = 0, npc = 0;
   std::vector<int64_t> perm;      // internal -> caller
   std::vector<int64_t> internal;  // caller -> internal
   std::vector<IoletHost> iolets;  // inlets then outlets
@@ -228,8 +228,11 @@ struct hlb_solver {
   DevBuf<uint32_t> src, binfo, b_wall, b_iolet;
   DevBuf<int32_t> b_io, io_iolet, io_bc, io_site, iolet_kind;
   DevBuf<float> b_q, iolet_nf;
-  DevBuf<double> io_stash, io_ladd, iolet_n, weights, mrt_c;
-  DevBuf<float> mrt_c_f;
+  DevBuf<double> io_stash, io_ladd, iolet_n, weights;
+  DevBuf<int32_t> bnbr;
+  DevBuf<uint64_t> wave_fluid, wave_bnd;
+  DevBuf<uint32_t> wave_rec, wave_cbase;
+  std::vector<int64_t> slot_of;   // compact -> state index (slot or compact)
   DevBuf<int64_t> d_perm;
   void* state[2] = {nullptr, nullptr};
   int cur = 0;
@@ -270,6 +273,8 @@ struct hlb_solver {
          b_io.bytes() + io_iolet.bytes() + io_bc.bytes() + io_site.bytes() +
          iolet_kind.bytes() + b_q.bytes() + iolet_nf.bytes() + io_stash.bytes() +
          io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
+         bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() + wave_rec.bytes() +
+         wave_cbase.bytes() +
          s_soa.bytes() + s_aos.bytes() + rho.bytes() + u.bytes() + u_snap.bytes();
     return b;
   }
@@ -302,12 +307,12 @@ struct hlb_solver {
   }
 
   int ensure_scratch_soa() {
-    if (!s_soa.p) HIP_OR_FAIL(s_soa.alloc((size_t)Q * np));
+    if (!s_soa.p) HIP_OR_FAIL(s_soa.alloc((size_t)Q * npc));
     return HLB_OK;
   }
   int ensure_moments() {
-    if (!rho.p) HIP_OR_FAIL(rho.alloc((size_t)np));
-    if (!u.p) HIP_OR_FAIL(u.alloc((size_t)3 * np));
+    if (!rho.p) HIP_OR_FAIL(rho.alloc((size_t)npc));
+    if (!u.p) HIP_OR_FAIL(u.alloc((size_t)3 * npc));
     return HLB_OK;
   }
 
@@ -379,6 +384,8 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD)
       return fail(HLB_E_INVALID, "unknown iolet streamer");
   if (P.storage < 0 || P.storage > 2) return fail(HLB_E_INVALID, "unknown storage");
+  if (P.layout != HLB_LAYOUT_BLOCKS && P.layout != HLB_LAYOUT_PULL_TABLE)
+    return fail(HLB_E_INVALID, "unknown layout");
   if (!(P.tau > 0.0) || !std::isfinite(P.tau)) return fail(HLB_E_INVALID, "tau must be positive");
   if (P.storage != HLB_STORE_FP64 && !(P.rho0 > 0.0))
     return fail(HLB_E_INVALID, "rho0 must be positive for reduced-precision storage");
@@ -396,8 +403,8 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   const int Q = P.lattice;
   s->Q = Q;
   s->n = n;
-  s->np = (n + 255) / 256 * 256;
-  const int64_t np = s->np;
+  s->npc = (n + 255) / 256 * 256;
+  const bool blocks_layout = P.layout == HLB_LAYOUT_BLOCKS;
   LatticeHost L(Q);
   const uint32_t all = (1u << (Q - 1)) - 1u;
 
@@ -453,9 +460,10 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
       for (int d = 1; d < Q; ++d)
         if ((wm >> (d - 1)) & 1u) {
           const float q = dm->cut_distance[i * (Q - 1) + d - 1];
-          if (!(q > 0.0f) || !std::isfinite(q))
+          // BFL divides by 2q only for q >= 0.5 (BouzidiFirdaousLallemand.h:56-66)
+          if (!std::isfinite(q))
             return fail(HLB_E_INVALID, "site " + std::to_string(i) +
-                                           ": BFL wall link needs a positive cut distance");
+                                           ": BFL wall link needs a finite cut distance");
         }
   }
 
@@ -504,8 +512,27 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     return it->second[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)];
   };
 
+  // Blocks of the compact order (it is sorted by Morton block key).
+  std::vector<int64_t> blk_of(n);
+  std::vector<uint64_t> blk_keys;
+  for (int64_t o = 0; o < n; ++o) {
+    const uint64_t k = keys[o].first >> 9;
+    if (blk_keys.empty() || blk_keys.back() != k) blk_keys.push_back(k);
+    blk_of[o] = (int64_t)blk_keys.size() - 1;
+  }
+  const int64_t nblocks = (int64_t)blk_keys.size();
+  s->np = blocks_layout ? nblocks * 512 : s->npc;
+  const int64_t np = s->np;
+  s->slot_of.resize(n);
+  for (int64_t o = 0; o < n; ++o)
+    s->slot_of[o] = blocks_layout ? ((blk_of[o] << 9) | (int64_t)(keys[o].first & 511)) : o;
+
   // Pull table, boundary and iolet records.
-  std::vector<uint32_t> h_src((size_t)(Q - 1) * np, 0), h_binfo(np, 0);
+  std::vector<uint32_t> h_src, h_binfo;
+  if (!blocks_layout) {
+    h_src.assign((size_t)(Q - 1) * np, 0);
+    h_binfo.assign(np, 0);
+  }
   std::vector<uint32_t> h_bwall, h_biolet;
   std::vector<int32_t> h_bio, h_io_iolet, h_io_bc, h_io_site;
   std::vector<float> h_bq;
@@ -534,11 +561,11 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
                         (own ? ": slot written both by a cut link and by a fluid neighbour"
                              : ": slot has no writer (non-fluid neighbour without a cut link)"));
       }
-      h_src[(size_t)(e - 1) * np + o] = own ? (uint32_t)o : (uint32_t)j;
+      if (!blocks_layout) h_src[(size_t)(e - 1) * np + o] = own ? (uint32_t)o : (uint32_t)j;
     }
     if (wm | im) {
       const uint32_t rec = (uint32_t)h_bwall.size();
-      h_binfo[o] = rec + 1;
+      if (!blocks_layout) h_binfo[o] = rec + 1;
       h_bwall.push_back(wm);
       h_biolet.push_back(im);
       for (int d = 1; d < Q; ++d)
@@ -577,6 +604,89 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     }
   }
 
+  // Block layout tables: neighbour blocks and per-wave masks.
+  std::vector<int32_t> h_bnbr;
+  std::vector<uint64_t> h_wf, h_wb;
+  std::vector<uint32_t> h_wr, h_wc;
+  if (blocks_layout) {
+    // Block neighbour table (offsets (dx,dy,dz) in {-1,0,1}^3, index 13 = self)
+    std::unordered_map<uint64_t, int32_t> ordinal;
+    ordinal.reserve(nblocks * 2);
+    std::vector<int64_t> bxyz(3 * nblocks);
+    for (int64_t o = 0; o < n; ++o) {
+      const int64_t i = s->perm[o];
+      const int64_t b = blk_of[o];
+      bxyz[3 * b] = (dm->coords[3 * i] - lo[0]) >> 3;
+      bxyz[3 * b + 1] = (dm->coords[3 * i + 1] - lo[1]) >> 3;
+      bxyz[3 * b + 2] = (dm->coords[3 * i + 2] - lo[2]) >> 3;
+    }
+    for (int64_t b = 0; b < nblocks; ++b)
+      ordinal[bkey(bxyz[3 * b], bxyz[3 * b + 1], bxyz[3 * b + 2])] = (int32_t)b;
+    h_bnbr.assign((size_t)nblocks * 27, -1);
+    for (int64_t b = 0; b < nblocks; ++b)
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dz = -1; dz <= 1; ++dz) {
+            const int64_t X = bxyz[3 * b] + dx, Y = bxyz[3 * b + 1] + dy, Z = bxyz[3 * b + 2] + dz;
+            if (X < 0 || Y < 0 || Z < 0) continue;
+            auto it = ordinal.find(bkey(X, Y, Z));
+            if (it != ordinal.end())
+              h_bnbr[(size_t)b * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] = it->second;
+          }
+    const int64_t nwaves = nblocks * 8;
+    if (nwaves >= (int64_t)0xFFFFFFFFll) return fail(HLB_E_INVALID, "domain too large");
+    h_wf.assign(nwaves, 0);
+    h_wb.assign(nwaves, 0);
+    h_wr.assign(nwaves, 0);
+    h_wc.assign(nwaves, 0);
+    uint32_t rec = 0;
+    int64_t o = 0;
+    for (int64_t w = 0; w < nwaves; ++w) {
+      h_wr[w] = rec;
+      h_wc[w] = (uint32_t)o;
+      while (o < n && (s->slot_of[o] >> 6) == w) {
+        const int lane = (int)(s->slot_of[o] & 63);
+        const int64_t i = s->perm[o];
+        h_wf[w] |= 1ull << lane;
+        if (dm->wall_mask[i] | dm->iolet_mask[i]) {
+          h_wb[w] |= 1ull << lane;
+          ++rec;
+        }
+        ++o;
+      }
+    }
+    s->dom.nwaves = (uint32_t)nwaves;
+    // Check, on the host, every neighbour the block-layout kernel will
+    // compute (BlockLayout::src in lbm_kernels.hip): each bulk pull of each
+    // fluid site must land on the slot of its source site.
+    for (int64_t o2 = 0; o2 < n; ++o2) {
+      const int64_t i = s->perm[o2];
+      const uint32_t cutm = dm->wall_mask[i] | dm->iolet_mask[i];
+      const int64_t slot = s->slot_of[o2];
+      const int64_t w = slot >> 6, lane = slot & 63;
+      const int lx = (int)(w & 7), ly = (int)(lane >> 3), lz = (int)(lane & 7);
+      const int64_t x = dm->coords[3 * i] - lo[0], y = dm->coords[3 * i + 1] - lo[1],
+                    z = dm->coords[3 * i + 2] - lo[2];
+      for (int e = 1; e < Q; ++e) {
+        if ((cutm >> (LatticeHost::inv(e) - 1)) & 1u) continue;
+        const int CX = L.c[e][0], CY = L.c[e][1], CZ = L.c[e][2];
+        const int sx = lx - CX;
+        const int ox = sx < 0 ? -1 : (sx > 7 ? 1 : 0);
+        const bool cy = CY != 0 && (CY > 0 ? ly == 0 : ly == 7);
+        const bool cz = CZ != 0 && (CZ > 0 ? lz == 0 : lz == 7);
+        const int32_t* nbt = &h_bnbr[(size_t)(w >> 3) * 27];
+        const int base = 13 + 9 * ox;
+        const int32_t nb = cy ? (cz ? nbt[base - 3 * CY - CZ] : nbt[base - 3 * CY])
+                              : (cz ? nbt[base - CZ] : nbt[base]);
+        const int64_t src = ((int64_t)nb << 9) +
+                            (((sx & 7) << 6) | (((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
+        const int64_t j = lookup(x - CX, y - CY, z - CZ);
+        if (nb < 0 || j < 0 || src != s->slot_of[j])
+          return fail(HLB_E_INVALID, "internal error: block neighbour table inconsistent");
+      }
+    }
+  }
+
   // Device set-up.
   HIP_OR_FAIL(hipSetDevice(P.device));
   HIP_OR_FAIL(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -586,6 +696,11 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   for (auto& st : s->state) HIP_OR_FAIL(hipMalloc(&st, sb * Q * np));
   HIP_OR_FAIL(s->src.upload(h_src));
   HIP_OR_FAIL(s->binfo.upload(h_binfo));
+  HIP_OR_FAIL(s->bnbr.upload(h_bnbr));
+  HIP_OR_FAIL(s->wave_fluid.upload(h_wf));
+  HIP_OR_FAIL(s->wave_bnd.upload(h_wb));
+  HIP_OR_FAIL(s->wave_rec.upload(h_wr));
+  HIP_OR_FAIL(s->wave_cbase.upload(h_wc));
   HIP_OR_FAIL(s->b_wall.upload(h_bwall));
   HIP_OR_FAIL(s->b_iolet.upload(h_biolet));
   HIP_OR_FAIL(s->b_io.upload(h_bio));
@@ -623,8 +738,15 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   }
 
   hlb::DevDomain& D = s->dom;
+  D.layout = blocks_layout ? 1 : 0;
   D.n = n;
   D.np = np;
+  D.npc = s->npc;
+  D.bnbr = s->bnbr.p;
+  D.wave_fluid = s->wave_fluid.p;
+  D.wave_bnd = s->wave_bnd.p;
+  D.wave_rec = s->wave_rec.p;
+  D.wave_cbase = s->wave_cbase.p;
   D.src = s->src.p;
   D.binfo = s->binfo.p;
   D.nb = (int64_t)h_bwall.size();
@@ -652,39 +774,8 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   A.omega_f = (float)A.omega;
   A.rho0 = P.storage == HLB_STORE_FP64 ? 0.0 : P.rho0;
   A.unstable = s->unstable.p;
-  if (P.kernel == HLB_MRT) {
-    // collisionMatrixDiagonals * normalisedReducedMomentBasis (MRT.h:96-102,139-152)
-    const int nm = Q == 15 ? 11 : 15;
-    double S[15];
-    if (Q == 15) {
-      const double v[11] = {1.6, 1.2, 1.6, 1.6, 1.6, 1.0 / P.tau, 1.0 / P.tau,
-                            1.0 / P.tau, 1.0 / P.tau, 1.0 / P.tau, 1.2};
-      std::copy(v, v + 11, S);
-    } else {
-      const double v[15] = {1.19, 1.4, 1.2, 1.2, 1.2, 1.0 / P.tau, 1.4, 1.0 / P.tau,
-                            1.4, 1.0 / P.tau, 1.0 / P.tau, 1.0 / P.tau, 1.98, 1.98, 1.98};
-      std::copy(v, v + 15, S);
-    }
-    std::vector<double> c((size_t)nm * Q);
-    std::vector<float> cf((size_t)nm * Q);
-    for (int k = 0; k < nm; ++k) {
-      double diag = 0.0;
-      for (int i = 0; i < Q; ++i) {
-        const double m = Q == 15 ? kM15[k][i] : kM19[k][i];
-        diag = diag + m * m;
-      }
-      for (int d = 0; d < Q; ++d) {
-        const double m = Q == 15 ? kM15[k][d] : kM19[k][d];
-        const double N = m / diag;
-        c[k * Q + d] = S[k] * N;
-        cf[k * Q + d] = (float)c[k * Q + d];
-      }
-    }
-    HIP_OR_FAIL(s->mrt_c.upload(c));
-    HIP_OR_FAIL(s->mrt_c_f.upload(cf));
-    A.mrt_c = s->mrt_c.p;
-    A.mrt_c_f = s->mrt_c_f.p;
-  }
+  // MRT relaxation rate of the viscous moments (DHumieres*MRTBasis.cc)
+  A.mrt_s_tau = 1.0 / P.tau;
   s->initialised = false;
   *out = s.release();
   return HLB_OK;
@@ -701,7 +792,7 @@ int hlb_set_equilibrium(hlb_solver* s, double density, const double momentum[3])
   feq_host(L, density, m, feq);
   DevBuf<double> vals;
   HIP_OR_FAIL(vals.upload(std::vector<double>(feq, feq + s->Q)));
-  if (hlb::launch_fill(s->args, s->n, s->np, vals.p, s->state[s->cur], s->stream))
+  if (hlb::launch_fill(s->dom, s->args, vals.p, s->state[s->cur], s->stream))
     return fail(HLB_E_DEVICE, "fill kernel launch failed");
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   s->post_stream = true;
@@ -781,7 +872,7 @@ int hlb_get_distributions(hlb_solver* s, double* f_out) {
   int r = s->post_stream_soa();
   if (r) return r;
   if (!s->s_aos.p) HIP_OR_FAIL(s->s_aos.alloc((size_t)s->Q * s->n));
-  if (hlb::launch_soa_to_aos(s->Q, s->n, s->np, s->s_soa.p, s->d_perm.p, s->s_aos.p, s->stream))
+  if (hlb::launch_soa_to_aos(s->Q, s->n, s->npc, s->s_soa.p, s->d_perm.p, s->s_aos.p, s->stream))
     return fail(HLB_E_DEVICE, "permute kernel launch failed");
   HIP_OR_FAIL(hipMemcpyAsync(f_out, s->s_aos.p, sizeof(double) * s->Q * s->n,
                              hipMemcpyDeviceToHost, s->stream));
@@ -794,8 +885,8 @@ int hlb_set_distributions(hlb_solver* s, const double* f_in) {
   if (!s->s_aos.p) HIP_OR_FAIL(s->s_aos.alloc((size_t)s->Q * s->n));
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   HIP_OR_FAIL(hipMemcpy(s->s_aos.p, f_in, sizeof(double) * s->Q * s->n, hipMemcpyHostToDevice));
-  if (hlb::launch_aos_to_state(s->args, s->n, s->np, s->s_aos.p, s->d_perm.p,
-                               s->state[s->cur], s->stream))
+  if (hlb::launch_import(s->dom, s->args, s->s_aos.p, s->d_perm.p, s->state[s->cur],
+                         s->stream))
     return fail(HLB_E_DEVICE, "import kernel launch failed");
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   s->post_stream = true;
@@ -808,7 +899,7 @@ static int compute_moments(hlb_solver* s) {
   if (r) return r;
   r = s->ensure_moments();
   if (r) return r;
-  if (hlb::launch_moments(s->Q, s->n, s->np, s->s_soa.p, s->rho.p, s->u.p, s->stream))
+  if (hlb::launch_moments(s->Q, s->n, s->npc, s->s_soa.p, s->rho.p, s->u.p, s->stream))
     return fail(HLB_E_DEVICE, "moments kernel launch failed");
   return HLB_OK;
 }
@@ -818,17 +909,17 @@ int hlb_get_macroscopic(hlb_solver* s, double* density, double* velocity) {
   if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
   int r = compute_moments(s);
   if (r) return r;
-  std::vector<double> rho(s->np), u(3 * s->np);
-  HIP_OR_FAIL(hipMemcpyAsync(rho.data(), s->rho.p, sizeof(double) * s->np,
+  std::vector<double> rho(s->npc), u(3 * s->npc);
+  HIP_OR_FAIL(hipMemcpyAsync(rho.data(), s->rho.p, sizeof(double) * s->npc,
                              hipMemcpyDeviceToHost, s->stream));
-  HIP_OR_FAIL(hipMemcpyAsync(u.data(), s->u.p, sizeof(double) * 3 * s->np,
+  HIP_OR_FAIL(hipMemcpyAsync(u.data(), s->u.p, sizeof(double) * 3 * s->npc,
                              hipMemcpyDeviceToHost, s->stream));
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   for (int64_t o = 0; o < s->n; ++o) {
     const int64_t c = s->perm[o];
     if (density) density[c] = rho[o];
     if (velocity)
-      for (int a = 0; a < 3; ++a) velocity[3 * c + a] = u[(size_t)a * s->np + o];
+      for (int a = 0; a < 3; ++a) velocity[3 * c + a] = u[(size_t)a * s->npc + o];
   }
   return HLB_OK;
 }
@@ -840,7 +931,7 @@ int hlb_reduce(hlb_solver* s, hlb_monitor* out) {
   if (r) return r;
   const int nb = 1024;
   if (!s->partials.p) HIP_OR_FAIL(s->partials.alloc(5 * nb));
-  if (hlb::launch_reduce(s->Q, s->n, s->np, s->s_soa.p, s->rho.p, s->u.p, s->partials.p, nb,
+  if (hlb::launch_reduce(s->Q, s->n, s->npc, s->s_soa.p, s->rho.p, s->u.p, s->partials.p, nb,
                          s->stream))
     return fail(HLB_E_DEVICE, "reduction kernel launch failed");
   std::vector<double> part(5 * nb);
@@ -855,15 +946,15 @@ int hlb_reduce(hlb_solver* s, hlb_monitor* out) {
       const int64_t o = s->io_sites_internal[k];
       slots[k] = o;                         // rho
       slots[ni + k] = o;                    // u_x
-      slots[2 * ni + k] = s->np + o;        // u_y
-      slots[3 * ni + k] = 2 * s->np + o;    // u_z
+      slots[2 * ni + k] = s->npc + o;       // u_y
+      slots[3 * ni + k] = 2 * s->npc + o;   // u_z
     }
     DevBuf<int64_t> d_slots;
     DevBuf<double> d_g;
     HIP_OR_FAIL(d_slots.upload(slots));
     HIP_OR_FAIL(d_g.alloc(4 * ni));
-    if (hlb::launch_gather_pairs(ni, s->np, d_slots.p, s->rho.p, 0, 0.0, nullptr, d_g.p, s->stream) ||
-        hlb::launch_gather_pairs(3 * ni, s->np, d_slots.p + ni, s->u.p, 0, 0.0, nullptr,
+    if (hlb::launch_gather_pairs(ni, s->npc, d_slots.p, s->rho.p, 0, 0.0, nullptr, d_g.p, s->stream) ||
+        hlb::launch_gather_pairs(3 * ni, s->npc, d_slots.p + ni, s->u.p, 0, 0.0, nullptr,
                                  d_g.p + ni, s->stream))
       return fail(HLB_E_DEVICE, "gather kernel launch failed");
     HIP_OR_FAIL(hipMemcpyAsync(g.data(), d_g.p, sizeof(double) * 4 * ni, hipMemcpyDeviceToHost,
@@ -900,8 +991,8 @@ int hlb_snapshot_velocity(hlb_solver* s) {
   if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
   int r = compute_moments(s);
   if (r) return r;
-  if (!s->u_snap.p) HIP_OR_FAIL(s->u_snap.alloc(3 * s->np));
-  HIP_OR_FAIL(hipMemcpyAsync(s->u_snap.p, s->u.p, sizeof(double) * 3 * s->np,
+  if (!s->u_snap.p) HIP_OR_FAIL(s->u_snap.alloc(3 * s->npc));
+  HIP_OR_FAIL(hipMemcpyAsync(s->u_snap.p, s->u.p, sizeof(double) * 3 * s->npc,
                              hipMemcpyDeviceToDevice, s->stream));
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   return HLB_OK;
@@ -915,7 +1006,7 @@ int hlb_velocity_change(hlb_solver* s, double ref, double* out) {
   if (r) return r;
   const int nb = 1024;
   if (!s->partials.p) HIP_OR_FAIL(s->partials.alloc(5 * nb));
-  if (hlb::launch_velocity_change(s->n, s->np, s->u.p, s->u_snap.p, 1.0 / ref, s->partials.p,
+  if (hlb::launch_velocity_change(s->n, s->npc, s->u.p, s->u_snap.p, 1.0 / ref, s->partials.p,
                                   nb, s->stream))
     return fail(HLB_E_DEVICE, "velocity change kernel launch failed");
   std::vector<double> part(nb);
@@ -938,7 +1029,7 @@ int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count, double* ho
     const int64_t site = pairs[2 * k], d = pairs[2 * k + 1];
     if (site < 0 || site >= s->n || d < 0 || d >= s->Q)
       return fail(HLB_E_INVALID, "halo pair out of range");
-    slots[k] = d * s->np + s->internal[site];
+    slots[k] = d * s->np + s->slot_of[s->internal[site]];
   }
   DevBuf<int64_t> d_slots;
   DevBuf<double> d_out;

@@ -229,6 +229,7 @@ def _iolet_array(lst):
 
 
 STORAGE = {"fp64": N.HLB_STORE_FP64, "fp32": N.HLB_STORE_FP32, "fp16": N.HLB_STORE_FP16}
+LAYOUT = {"blocks": N.HLB_LAYOUT_BLOCKS, "pull_table": N.HLB_LAYOUT_PULL_TABLE}
 
 
 class LBM:
@@ -236,7 +237,7 @@ class LBM:
 
     def __init__(self, traits: Traits, domain: Domain, params: LbmParameters,
                  inlets=(), outlets=(), storage: str = "fp64", rho0: float = 1.0,
-                 device: int = 0):
+                 device: int = 0, layout: str = "blocks"):
         if domain.Q != traits.Lattice.NUMVECTORS:
             raise ValueError("domain was projected for a different lattice")
         self.traits, self.domain, self.params = traits, domain, params
@@ -251,6 +252,7 @@ class LBM:
         p.tau = params.GetTau()
         p.rho0 = rho0
         p.device = device
+        p.layout = LAYOUT[layout]
         self._inlets, self._outlets = list(inlets), list(outlets)
         self._cdom = domain.as_c()
         h = C.c_void_p()

@@ -67,6 +67,12 @@ enum hlb_site_type { HLB_SITE_FLUID = 1, HLB_SITE_INLET = 2, HLB_SITE_OUTLET = 3
  * FP16 store f_i - w_i*rho0 and collide in binary32; iolet boundary values
  * and all reductions stay binary64. */
 enum hlb_storage { HLB_STORE_FP64 = 0, HLB_STORE_FP32 = 1, HLB_STORE_FP16 = 2 };
+/* Layout of the distributions in HBM.  HLB_LAYOUT_BLOCKS gives every 8^3
+ * block that holds fluid 512 slots (Morton block order, z fastest) and finds
+ * neighbours by slot arithmetic; HLB_LAYOUT_PULL_TABLE packs the fluid sites
+ * and reads a 32-bit pull index per site and direction (smaller for very
+ * sparse geometries).  Results are identical. */
+enum hlb_layout { HLB_LAYOUT_BLOCKS = 0, HLB_LAYOUT_PULL_TABLE = 1 };
 
 typedef struct hlb_solver hlb_solver;
 
@@ -81,7 +87,7 @@ typedef struct {
   double rho0;         /* reference density for FP32/FP16 storage (ignored
                           for FP64) */
   int32_t device;      /* HIP device ordinal */
-  int32_t reserved;
+  int32_t layout;      /* hlb_layout */
 } hlb_params;
 
 typedef struct {

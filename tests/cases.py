@@ -67,11 +67,11 @@ class Case:
         return L.Traits(L.LATTICES[self.domain.Q], KERNELS[self.kernel][1],
                         WALLS[self.wall][1], BCS[self.inlet_bc][1], BCS[self.outlet_bc][1])
 
-    def product(self, storage="fp64", device=0) -> L.LBM:
+    def product(self, storage="fp64", device=0, layout="blocks") -> L.LBM:
         lbm = L.LBM(self.traits(), self.domain, L.LbmParameters(tau=self.tau),
                     [i.to_product() for i in self.inlets],
                     [o.to_product() for o in self.outlets], storage=storage,
-                    rho0=self.rho0, device=device)
+                    rho0=self.rho0, device=device, layout=layout)
         lbm.SetInitialConditions(self.rho0)
         return lbm
 

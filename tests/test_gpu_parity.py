@@ -11,11 +11,12 @@ from tests import cases as K
 pytestmark = pytest.mark.gpu
 
 
-def _run_and_compare(case: K.Case, steps, chunks=None, label=""):
-    print(f"[parity] {label} n={case.domain.n} Q={case.domain.Q} {case.kernel} "
-          f"{case.wall} {case.inlet_bc}/{case.outlet_bc} steps={steps}", flush=True)
+def _run_and_compare(case: K.Case, steps, chunks=None, label="", layout="blocks"):
+    print(f"[parity] {label} layout={layout} n={case.domain.n} Q={case.domain.Q} "
+          f"{case.kernel} {case.wall} {case.inlet_bc}/{case.outlet_bc} steps={steps}",
+          flush=True)
     orc = case.oracle()
-    gpu = case.product()
+    gpu = case.product(layout=layout)
     np.testing.assert_array_equal(gpu.get_distributions(), orc.get_f())
     done = 0
     for n in (chunks or [steps]):
@@ -49,9 +50,10 @@ PARITY_CASES = [
 ]
 
 
+@pytest.mark.parametrize("layout", ["blocks", "pull_table"])
 @pytest.mark.parametrize("label,make,steps", PARITY_CASES, ids=[c[0] for c in PARITY_CASES])
-def test_parity_bitwise(label, make, steps):
-    _run_and_compare(make(), steps, chunks=[1, 2, steps - 3], label=label)
+def test_parity_bitwise(label, make, steps, layout):
+    _run_and_compare(make(), steps, chunks=[1, 2, steps - 3], label=label, layout=layout)
 
 
 def test_macroscopic_and_monitor_match_oracle():
@@ -112,22 +114,23 @@ def test_convergence_monitor():
     np.testing.assert_allclose(gpu.velocity_change(0.01), expect, rtol=1e-14)
 
 
-def test_halo_pack_returns_post_collision_values():
-    case = K.cylinder_case(5.0, 16, 19, "LBGK", "SBB", "cosine")
-    orc, gpu = _run_and_compare(case, 4, label="halo")
-    # Post-collision values leaving site s along d are what the oracle's next
-    # step pushes into the neighbour: compare through one oracle step.
-    dom = case.domain
-    rng = np.random.default_rng(0)
-    sites = rng.choice(dom.n, 50, replace=False)
-    pairs = np.array([(s, d) for s in sites for d in range(19)], np.int64)
-    packed = gpu.halo_pack(pairs)
-    f_now = orc.get_f()
+@pytest.mark.parametrize("layout", ["blocks", "pull_table"])
+def test_halo_pack_returns_pending_post_collision_values(layout):
+    """After n steps the state holds the post-collision values of step n whose
+    streaming is still pending: they equal the collision of the oracle's state
+    before its n-th step, i.e. what the reference sends in step n."""
     from oracle import bindings as O
+    case = K.cylinder_case(5.0, 16, 19, "LBGK", "SBB", "cosine")
+    orc, gpu = case.oracle(), case.product(layout=layout)
+    orc.step(3)
+    gpu.step(4)
+    f3 = orc.get_f()          # fOld at the start of step 4
+    rng = np.random.default_rng(0)
+    sites = rng.choice(case.domain.n, 50, replace=False)
+    pairs = np.array([(s, d) for s in sites for d in range(19)], np.int64)
+    packed = gpu.halo_pack(pairs).reshape(len(sites), 19)
     fpc = np.empty(19)
-    import ctypes
-    for k, (s, d) in enumerate(pairs):
-        if d == 0 or k % 19 == 0:
-            O.lib().orc_collide_lbgk(19, case.tau, O._p(np.ascontiguousarray(f_now[s])),
-                                     O._p(fpc), None, None, None)
-        assert packed[k] == fpc[d]
+    for k, s in enumerate(sites):
+        O.lib().orc_collide_lbgk(19, case.tau, O._p(np.ascontiguousarray(f3[s])),
+                                 O._p(fpc), None, None, None)
+        np.testing.assert_array_equal(packed[k], fpc)
