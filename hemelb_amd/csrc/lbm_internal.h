@@ -28,18 +28,19 @@ struct DevDomain {
   const uint64_t* wave_bnd = nullptr;    // [nwaves] lane has a boundary record
   const uint32_t* wave_rec = nullptr;    // [nwaves] first boundary record
   const uint32_t* wave_cbase = nullptr;  // [nwaves] compact index of first fluid lane
-  // Pull table: src[(d-1)*np + x] is the site whose post-collision value of
-  // direction d streams into slot d of site x (the site at x - c_d).  For a
-  // slot fed by a wall or iolet link it holds x itself (never dereferenced
-  // as a neighbour).
-  const uint32_t* src = nullptr;
+  const uint32_t* slot_cut = nullptr;    // [np] wall|iolet link mask per slot
+  // Pull table (layout 0): src[(d-1)*np + x] is the offset inside array d
+  // of the value streaming into slot d of site x: the site at x - c_d, or
+  // x + (inv(d) - d) * np (the site's own inverse slot) for a cut link.
+  const int32_t* src = nullptr;
   // 0 for a site with no wall/iolet link, else its boundary record + 1.
   const uint32_t* binfo = nullptr;
   int64_t nb = 0;                  // boundary records
   const uint32_t* b_wall = nullptr;   // [nb] wall link mask
   const uint32_t* b_iolet = nullptr;  // [nb] iolet link mask
   const int32_t* b_io = nullptr;      // [nb] iolet-site record or -1
-  const float* b_q = nullptr;         // [nb][Q-1] cut distances
+  const float* b_q = nullptr;         // [nb][QS] cut distances of links 1..Q-1,
+                                      // QS = Q-1 rounded up to a multiple of 4
   int64_t ni = 0;                  // iolet-site records
   const int32_t* io_iolet = nullptr;  // [ni] global iolet index (inlets first)
   const int32_t* io_bc = nullptr;     // [ni] hlb_iolet_bc of the site's group

@@ -326,10 +326,11 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
 // ---------------------------------------------------------------------------
 struct SiteRef {
   int64_t x;      // index into every per-direction array
-  uint32_t rec;   // boundary record (valid when boundary)
-  bool boundary;
+  uint32_t cut;   // wall|iolet link mask of the site (0: bulk site)
+  uint32_t rec;   // boundary record (valid when cut != 0)
   uint32_t w;     // block layout: wave index (block * 8 + x plane)
   uint32_t lane;  // block layout: lane = y * 8 + z inside the plane
+  int32_t tb[27]; // block layout: neighbour blocks of this wave's block
 };
 
 __device__ __forceinline__ uint32_t logical_workgroup() {
@@ -342,19 +343,22 @@ __device__ __forceinline__ uint32_t logical_workgroup() {
   return start + k;
 }
 
-// Compact layout: sites packed in Morton-block order, 32-bit pull table.
+// Compact layout: sites packed in Morton-block order.  The 32-bit pull
+// table holds, per direction e, the offset (inside array e) of the value
+// that streams into slot e: the source site for a fluid link, or the site's
+// own slot of the inverse direction (x + (inv(e) - e) * np) for a cut link.
 struct SparseLayout {
   static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
     r.x = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
     if (r.x >= D.n) return false;
     const uint32_t bi = D.binfo[r.x];
-    r.boundary = bi != 0;
+    r.cut = bi != 0 ? 1u : 0u;  // only "is boundary" is needed here
     r.rec = bi - 1;
     return true;
   }
   template <int Q, int E>
-  static __device__ __forceinline__ int64_t src(const DevDomain& D, const SiteRef& r) {
-    return D.src[(int64_t)(E - 1) * D.np + r.x];
+  static __device__ __forceinline__ int64_t offset(const DevDomain& D, const SiteRef& r) {
+    return (int64_t)D.src[(int64_t)(E - 1) * D.np + r.x];
   }
   static __device__ __forceinline__ int64_t compact(const DevDomain&, const SiteRef& r) {
     return r.x;
@@ -363,8 +367,9 @@ struct SparseLayout {
 
 // Block layout: every 8^3 block holding fluid owns 512 slots (x-plane major,
 // z fastest); a wave covers one x plane of one block.  Neighbours come from
-// slot arithmetic and the block's 27-entry neighbour table; per-wave 64-bit
-// masks say which lanes are fluid and which carry a boundary record.
+// slot arithmetic and the block's 27-entry neighbour table, which each wave
+// loads once; per-wave 64-bit masks say which lanes are fluid and which carry
+// a boundary record, and a per-slot mask marks the cut links.
 struct BlockLayout {
   static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
     const int64_t slot = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
@@ -373,33 +378,42 @@ struct BlockLayout {
     r.x = ((int64_t)r.w << 6) + r.lane;
     if (r.w >= D.nwaves) return false;
     const uint64_t fm = D.wave_fluid[r.w];
-    if (!((fm >> r.lane) & 1ull)) return false;
     const uint64_t bm = D.wave_bnd[r.w];
-    r.boundary = (bm >> r.lane) & 1ull;
-    r.rec = D.wave_rec[r.w] + (uint32_t)__popcll(bm & ((1ull << r.lane) - 1ull));
+    const uint32_t rb = D.wave_rec[r.w];
+    if (!((fm >> r.lane) & 1ull)) return false;
+    const int32_t* nbt = D.bnbr + (int64_t)(r.w >> 3) * 27;
+    sfor<0, 27>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      r.tb[i] = nbt[i];
+    });
+    // Only waves holding a boundary site need the per-slot cut masks.
+    r.cut = bm ? D.slot_cut[r.x] : 0u;
+    r.rec = rb + (uint32_t)__popcll(bm & ((1ull << r.lane) - 1ull));
     return true;
   }
   template <int Q, int E>
-  static __device__ __forceinline__ int64_t src(const DevDomain& D, const SiteRef& r) {
+  static __device__ __forceinline__ int64_t offset(const DevDomain& D, const SiteRef& r) {
     constexpr int CX = Lat<Q>::c[E][0], CY = Lat<Q>::c[E][1], CZ = Lat<Q>::c[E][2];
     const int lx = (int)(r.w & 7u);
     const int ly = (int)(r.lane >> 3), lz = (int)(r.lane & 7u);
-    // source = x - c_E; the wave-uniform x shift may leave the block
-    const int sx = lx - CX;
-    const int ox = sx < 0 ? -1 : (sx > 7 ? 1 : 0);
-    const bool cy = CY != 0 && (CY > 0 ? ly == 0 : ly == 7);
-    const bool cz = CZ != 0 && (CZ > 0 ? lz == 0 : lz == 7);
-    const int32_t* nbt = D.bnbr + (int64_t)(r.w >> 3) * 27;
-    const int base = 13 + 9 * ox;
-    const int32_t t00 = nbt[base];
-    int32_t nb = t00;
-    if constexpr (CY != 0 || CZ != 0) {
-      const int32_t t10 = nbt[base - 3 * CY];
-      const int32_t t01 = nbt[base - CZ];
-      const int32_t t11 = nbt[base - 3 * CY - CZ];
-      nb = cy ? (cz ? t11 : t10) : (cz ? t01 : t00);
-    }
-    return ((int64_t)nb << 9) + (((sx & 7) << 6) | (((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
+    // source = x - c_E; crossing a block face moves to the neighbour block
+    // at offset -c_E along that axis.
+    const bool bx = CX != 0 && (CX > 0 ? lx == 0 : lx == 7);  // wave-uniform
+    const bool by = CY != 0 && (CY > 0 ? ly == 0 : ly == 7);
+    const bool bz = CZ != 0 && (CZ > 0 ? lz == 0 : lz == 7);
+    constexpr int i000 = 13, i100 = 13 - 9 * CX, i010 = 13 - 3 * CY, i001 = 13 - CZ;
+    constexpr int i110 = 13 - 9 * CX - 3 * CY, i101 = 13 - 9 * CX - CZ;
+    constexpr int i011 = 13 - 3 * CY - CZ, i111 = 13 - 9 * CX - 3 * CY - CZ;
+    const int32_t n00 = bx ? r.tb[i100] : r.tb[i000];
+    const int32_t n10 = bx ? r.tb[i110] : r.tb[i010];
+    const int32_t n01 = bx ? r.tb[i101] : r.tb[i001];
+    const int32_t n11 = bx ? r.tb[i111] : r.tb[i011];
+    const int32_t nb = by ? (bz ? n11 : n10) : (bz ? n01 : n00);
+    const int64_t src = ((int64_t)nb << 9) +
+                        ((((lx - CX) & 7) << 6) | (((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
+    constexpr int DP = inv_dir(E);
+    const int64_t own = r.x + (int64_t)(DP - E) * D.np;
+    return ((r.cut >> (DP - 1)) & 1u) ? own : src;
   }
   static __device__ __forceinline__ int64_t compact(const DevDomain& D, const SiteRef& r) {
     const uint64_t fm = D.wave_fluid[r.w];
@@ -460,6 +474,9 @@ __device__ __forceinline__ double ladd_correction(const DevDomain& D,
 
 // Post-stream distributions of a site (the reference's fNew after
 // StreamAndCollide and PostStep) from the post-collision state `fin`.
+// All Q loads are issued first: a cut link reads the site's own value of the
+// inverse direction (the bounce-back value); the wall and iolet formulas are
+// applied afterwards.
 template <int Q, typename S, typename T, class Lay>
 __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
                                        const S* __restrict__ fin, const SiteRef& r,
@@ -467,49 +484,64 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
   using St = Store<S>;
   const int64_t np = D.np, x = r.x;
   f[0] = St::ld(fin + x);
-  if (!r.boundary) {
-    sfor<1, Q>([&](auto ei) {
-      constexpr int e = decltype(ei)::value;
-      f[e] = St::ld(fin + (int64_t)e * np + Lay::template src<Q, e>(D, r));
-    });
-    return;
-  }
-  const uint32_t rec = r.rec;
-  const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
-  const int io = D.b_io[rec];
-  const int k = io >= 0 ? D.io_iolet[io] : 0;
-  const int bc = io >= 0 ? D.io_bc[io] : 0;
-  const float* q = D.b_q + (int64_t)rec * (Q - 1);
   sfor<1, Q>([&](auto ei) {
     constexpr int e = decltype(ei)::value;
-    constexpr int dp = inv_dir(e);  // link of x that points at the source
-    constexpr uint32_t bit = 1u << (dp - 1);
-    if (im & bit) {
-      if (bc == 0) {
-        double v = nash_value<Q, e>(D, A, io, k);
-        if constexpr (kShifted<S>) v -= Lat<Q>::w[e] * A.rho0;
-        f[e] = (T)v;
-      } else {
-        f[e] = St::ld(fin + (int64_t)dp * np + x) -
-               (T)ladd_correction<Q, dp>(D, A, io, k);
-      }
-    } else if (wm & bit) {
-      const T own = St::ld(fin + (int64_t)dp * np + x);
-      if (A.wall == 0) {
-        f[e] = own;  // BounceBackLink::StreamLink
-      } else {
-        const T qd = (T)(double)q[dp - 1];
-        if ((wm & (1u << (e - 1))) || qd < T(0.5)) {
-          f[e] = own;
+    f[e] = St::ld(fin + (int64_t)e * np + Lay::template offset<Q, e>(D, r));
+  });
+  if (!r.cut) return;
+  const uint32_t rec = r.rec;
+  const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
+  // cut distances of links 1..Q-1, padded to a multiple of 4 floats
+  constexpr int QS = (Q - 1 + 3) / 4 * 4;
+  float q[QS];
+  if (A.wall == 1 && wm) {
+    const float4* q4 = reinterpret_cast<const float4*>(D.b_q + (int64_t)rec * QS);
+    sfor<0, QS / 4>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      const float4 v = q4[i];
+      q[4 * i] = v.x;
+      q[4 * i + 1] = v.y;
+      q[4 * i + 2] = v.z;
+      q[4 * i + 3] = v.w;
+    });
+    // BouzidiFirdaousLallemandLink::StreamLink (BouzidiFirdaousLallemand.h:41-66):
+    // with q >= 0.5 and a fluid opposite link the slot gets
+    // (f*_d + (2q - 1) f*_inv(d)) / (2q); both values are the site's own.
+    T opp[Q];
+    sfor<1, Q>([&](auto ei) {
+      constexpr int e = decltype(ei)::value;
+      constexpr int dp = inv_dir(e);
+      const bool need = ((wm >> (dp - 1)) & 1u) && !((im >> (dp - 1)) & 1u) &&
+                        !((wm >> (e - 1)) & 1u);
+      opp[e] = St::ld(fin + (need ? (int64_t)e * np + x : (int64_t)dp * np + x));
+    });
+    sfor<1, Q>([&](auto ei) {
+      constexpr int e = decltype(ei)::value;
+      constexpr int dp = inv_dir(e);
+      const T qd = (T)(double)q[dp - 1];
+      if (((wm >> (dp - 1)) & 1u) && !((im >> (dp - 1)) & 1u) && !((wm >> (e - 1)) & 1u) &&
+          !(qd < T(0.5)))
+        f[e] = (f[e] + (T(2.0) * qd - T(1)) * opp[e]) / (T(2.0) * qd);
+    });
+  }
+  if (im) {
+    const int io = D.b_io[rec];
+    const int k = D.io_iolet[io];
+    const int bc = D.io_bc[io];
+    sfor<1, Q>([&](auto ei) {
+      constexpr int e = decltype(ei)::value;
+      constexpr int dp = inv_dir(e);
+      if ((im >> (dp - 1)) & 1u) {
+        if (bc == 0) {
+          double v = nash_value<Q, e>(D, A, io, k);
+          if constexpr (kShifted<S>) v -= Lat<Q>::w[e] * A.rho0;
+          f[e] = (T)v;
         } else {
-          const T opp = St::ld(fin + (int64_t)e * np + x);
-          f[e] = (own + (T(2.0) * qd - T(1)) * opp) / (T(2.0) * qd);
+          f[e] = f[e] - (T)ladd_correction<Q, dp>(D, A, io, k);
         }
       }
-    } else {
-      f[e] = St::ld(fin + (int64_t)e * np + Lay::template src<Q, e>(D, r));
-    }
-  });
+    });
+  }
   if (A.wall == 1 && wm) {
     // BouzidiFirdaousLallemandLink::PostStepLink
     sfor<1, Q>([&](auto di) {
@@ -527,7 +559,7 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
 template <int Q, typename T>
 __device__ __forceinline__ void update_stash(const DevDomain& D, const SiteRef& r,
                                              const Hydro<Q, T>& h) {
-  if (!r.boundary) return;
+  if (!r.cut) return;
   const int io = D.b_io[r.rec];
   if (io < 0) return;
   const double rho = (double)h.rho;

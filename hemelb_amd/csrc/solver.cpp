@@ -225,7 +225,8 @@ struct hlb_solver {
   std::vector<IoletHost> iolets;  // inlets then outlets
   int n_inlets = 0;
   // device data
-  DevBuf<uint32_t> src, binfo, b_wall, b_iolet;
+  DevBuf<int32_t> src;
+  DevBuf<uint32_t> binfo, b_wall, b_iolet, slot_cut;
   DevBuf<int32_t> b_io, io_iolet, io_bc, io_site, iolet_kind;
   DevBuf<float> b_q, iolet_nf;
   DevBuf<double> io_stash, io_ladd, iolet_n, weights;
@@ -273,7 +274,8 @@ struct hlb_solver {
          b_io.bytes() + io_iolet.bytes() + io_bc.bytes() + io_site.bytes() +
          iolet_kind.bytes() + b_q.bytes() + iolet_nf.bytes() + io_stash.bytes() +
          io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
-         bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() + wave_rec.bytes() +
+         bnbr.bytes() + slot_cut.bytes() + wave_fluid.bytes() + wave_bnd.bytes() +
+         wave_rec.bytes() +
          wave_cbase.bytes() +
          s_soa.bytes() + s_aos.bytes() + rho.bytes() + u.bytes() + u_snap.bytes();
     return b;
@@ -528,10 +530,15 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     s->slot_of[o] = blocks_layout ? ((blk_of[o] << 9) | (int64_t)(keys[o].first & 511)) : o;
 
   // Pull table, boundary and iolet records.
-  std::vector<uint32_t> h_src, h_binfo;
+  std::vector<int32_t> h_src;
+  std::vector<uint32_t> h_binfo, h_cut;
   if (!blocks_layout) {
+    if (np >= ((int64_t)1 << 30))
+      return fail(HLB_E_INVALID, "too many sites for the pull-table layout");
     h_src.assign((size_t)(Q - 1) * np, 0);
     h_binfo.assign(np, 0);
+  } else {
+    h_cut.assign(np, 0);
   }
   std::vector<uint32_t> h_bwall, h_biolet;
   std::vector<int32_t> h_bio, h_io_iolet, h_io_bc, h_io_site;
@@ -561,15 +568,21 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
                         (own ? ": slot written both by a cut link and by a fluid neighbour"
                              : ": slot has no writer (non-fluid neighbour without a cut link)"));
       }
-      if (!blocks_layout) h_src[(size_t)(e - 1) * np + o] = own ? (uint32_t)o : (uint32_t)j;
+      // offset inside array e: the source site, or the own slot of the
+      // inverse direction (x + (inv(e) - e) * np) for a cut link
+      if (!blocks_layout)
+        h_src[(size_t)(e - 1) * np + o] = own ? (int32_t)(o + (int64_t)(dp - e) * np) : (int32_t)j;
     }
+    if (blocks_layout) h_cut[s->slot_of[o]] = wm | im;
     if (wm | im) {
       const uint32_t rec = (uint32_t)h_bwall.size();
       if (!blocks_layout) h_binfo[o] = rec + 1;
       h_bwall.push_back(wm);
       h_biolet.push_back(im);
-      for (int d = 1; d < Q; ++d)
-        h_bq.push_back(dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1] : -1.0f);
+      const int qs = (Q - 1 + 3) / 4 * 4;  // record stride: 16-byte aligned
+      for (int d = 1; d <= qs; ++d)
+        h_bq.push_back(d < Q && dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1]
+                                                 : -1.0f);
       if (im) {
         const int g = dm->site_type[i] == HLB_SITE_OUTLET ? 1 : 0;
         const int k = (g ? n_inlets : 0) + dm->iolet_id[i];
@@ -697,6 +710,7 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   HIP_OR_FAIL(s->src.upload(h_src));
   HIP_OR_FAIL(s->binfo.upload(h_binfo));
   HIP_OR_FAIL(s->bnbr.upload(h_bnbr));
+  HIP_OR_FAIL(s->slot_cut.upload(h_cut));
   HIP_OR_FAIL(s->wave_fluid.upload(h_wf));
   HIP_OR_FAIL(s->wave_bnd.upload(h_wb));
   HIP_OR_FAIL(s->wave_rec.upload(h_wr));
@@ -747,6 +761,7 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   D.wave_bnd = s->wave_bnd.p;
   D.wave_rec = s->wave_rec.p;
   D.wave_cbase = s->wave_cbase.p;
+  D.slot_cut = s->slot_cut.p;
   D.src = s->src.p;
   D.binfo = s->binfo.p;
   D.nb = (int64_t)h_bwall.size();
