@@ -8,14 +8,19 @@ distances, a Ladd/Womersley root inlet with a 0.05 peak centreline velocity
 and 16 Nash zero-gauge pressure outlets.
 
 A step is one full LB timestep over every fluid site, inputs resident in HBM.
-Prints one JSON line: MLUPS, the dominant kernel's HBM roofline and the CPU
-oracle's rate on a bounded sample of the same workload.
+Prints one JSON line: MLUPS, the dominant kernel's HBM roofline, the rates of
+the reduced-precision storage forms on the same workload and the CPU oracle's
+rate on a bounded sample of it.
+
+``--precision-report`` instead runs the small cylinder and the vessel tree
+(LBGK and MRT) to their full step counts in every storage form and prints one
+JSON line per (workload, form) with the velocity, pressure and wall shear
+stress errors against the binary64 run.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -31,40 +36,6 @@ DTYPE = {"fp64": "f64", "fp32": "f32", "fp16": "f16"}
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def tree_config(storage: str):
-    import numpy as np
-
-    import hemelb_amd as H
-
-    t0 = time.time()
-    g = H.Geometry.vessel_tree(0, 5, 40.0)
-    dom = g.project(19)
-    log(f"[bench] geometry: {dom.n} fluid sites, {len(dom.outlets)} outlets, "
-        f"{time.time() - t0:.1f}s")
-    tau, period, peak = 0.7, 5000.0, 0.05
-    nu = (tau - 0.5) / 3.0
-    inl = dom.inlets[0]
-    omega = 2 * math.pi / period
-    alpha = inl.radius * math.sqrt(omega / nu)
-    z = complex(-math.sqrt(0.5), math.sqrt(0.5)) * alpha
-    s, term, k = 1 + 0j, 1 + 0j, 0
-    while abs(term) > 1e-17 * abs(s) or k < 5:
-        k += 1
-        term *= -(z * z) / (4 * k * k)
-        s += term
-    amp = abs(1 - 1 / s) / omega
-    inlets = [H.InOutLetWomersleyVelocity(tuple(inl.position), tuple(inl.normal),
-                                          radius=inl.radius,
-                                          pressure_gradient_amplitude=peak / amp,
-                                          period=period, womersley_number=alpha)]
-    outlets = [H.InOutLetCosine(tuple(o.position), tuple(o.normal), density_mean=1.0,
-                                density_amp=0.0, period=period) for o in dom.outlets]
-    traits = H.Traits(H.D3Q19, H.LBGK, H.BouzidiFirdaousLallemand, H.LaddIolet,
-                      H.NashZerothOrderPressure)
-    return dom, traits, tau, inlets, outlets, dict(period=period, womersley=alpha,
-                                                   pressure_gradient=peak / amp)
 
 
 def cpu_baseline(dom, tau, inlets, outlets, steps):
@@ -89,6 +60,51 @@ def cpu_baseline(dom, tau, inlets, outlets, steps):
     return dom.n * steps / dt / 1e6, dt
 
 
+def timed_rate(lbm, n_sites, steps, warmup):
+    lbm.step(warmup)
+    lbm.synchronize()
+    ms = lbm.time_steps(steps)
+    lbm.synchronize()
+    return ms / steps, n_sites / (ms / steps * 1e-3) / 1e6
+
+
+def precision_report(args) -> int:
+    import numpy as np
+
+    from hemelb_amd import extraction as X
+    from hemelb_amd import workloads as WL
+
+    makers = {"small_cylinder": WL.small_cylinder,
+              "vessel_tree_lbgk": lambda: WL.vessel_tree("LBGK"),
+              "vessel_tree_mrt": lambda: WL.vessel_tree("MRT")}
+    names = args.report_workloads.split(",") if args.report_workloads else list(makers)
+    for name in names:
+        w = makers[name]()
+        steps = args.report_steps or w.steps
+        ref = None
+        for storage in ("fp64", "fp32", "fp16"):
+            lbm = w.solver(storage=storage)
+            t0 = time.perf_counter()
+            ms = lbm.time_steps(steps)
+            lbm.synchronize()
+            wall = time.perf_counter() - t0
+            fields = X.extract(lbm)
+            mon = lbm.monitor()
+            del lbm
+            rec = {"workload": w.name, "storage": storage, "steps": steps,
+                   "fluid_sites": w.domain.n, "ms_per_step": round(ms / steps, 5),
+                   "mlups": round(w.domain.n * steps / (ms * 1e-3) / 1e6, 1),
+                   "wall_s": round(wall, 2), "stable": mon["stable"],
+                   "mass": mon["mass"]}
+            if ref is None:
+                ref = fields
+                rec["velocity_max"] = float(np.sqrt((fields.velocity ** 2).sum(1)).max())
+            else:
+                rec["errors_vs_fp64"] = X.field_errors(fields, ref)
+            print(json.dumps(rec), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,19 +114,31 @@ def main() -> int:
     ap.add_argument("--layout", choices=["blocks", "pull_table"], default="blocks")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip timing the reduced-precision storage forms")
+    ap.add_argument("--form-steps", type=int, default=500)
+    ap.add_argument("--precision-report", action="store_true")
+    ap.add_argument("--report-steps", type=int, default=0,
+                    help="steps per precision-report run (default: the workload's)")
+    ap.add_argument("--report-workloads", default="")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "path is not part of this build.", file=sys.stderr)
         return 2
 
-    import hemelb_amd as H
+    if args.precision_report:
+        return precision_report(args)
 
-    dom, traits, tau, inlets, outlets, extra = tree_config(args.storage)
+    from hemelb_amd import workloads as WL
+
     t0 = time.time()
-    lbm = H.LBM(traits, dom, H.LbmParameters(tau=tau), inlets, outlets,
-                storage=args.storage, rho0=1.0, layout=args.layout)
-    lbm.SetInitialConditions(1.0)
+    work = WL.vessel_tree("LBGK")
+    dom, tau, inlets, outlets = work.domain, work.tau, work.inlets, work.outlets
+    log(f"[bench] geometry: {dom.n} fluid sites, {len(dom.outlets)} outlets, "
+        f"{time.time() - t0:.1f}s")
+    t0 = time.time()
+    lbm = work.solver(storage=args.storage, layout=args.layout)
     lbm.synchronize()
     setup_s = time.time() - t0
     log(f"[bench] solver set-up {setup_s:.1f}s, device memory "
@@ -141,6 +169,23 @@ def main() -> int:
             pass
     log(f"[bench] {args.steps} steps: {ms:.2f} ms on the solver stream "
         f"({wall:.3f}s wall), {mlups:.1f} MLUPS, {achieved:.0f} GB/s algorithmic")
+    del lbm
+    forms = None
+    if not args.no_forms:
+        forms = {}
+        for st in sorted(STORE_BYTES):
+            if st == args.storage:
+                continue
+            other = work.solver(storage=st, layout=args.layout)
+            f_ms, f_mlups = timed_rate(other, n, args.form_steps, args.warmup)
+            f_ach = 2 * 19 * STORE_BYTES[st] * n / (f_ms * 1e-3) / 1e9
+            forms[st] = {"value": round(f_mlups, 2), "unit": "MLUPS",
+                         "ms_per_step": round(f_ms, 5), "steps": args.form_steps,
+                         "ratio_to_headline": round(f_mlups / mlups, 3),
+                         "achieved_GBs": round(f_ach, 1)}
+            del other
+            log(f"[bench] {st}: {f_mlups:.1f} MLUPS ({f_mlups / mlups:.2f}x), "
+                f"{f_ach:.0f} GB/s algorithmic")
     cpu = None
     if not args.no_cpu_baseline:
         c_mlups, c_s = cpu_baseline(dom, tau, inlets, outlets, args.cpu_steps)
@@ -158,8 +203,10 @@ def main() -> int:
                    "walls": "BFL", "inlet": "Ladd/Womersley", "outlets": len(outlets),
                    "storage": args.storage, "layout": args.layout,
                    "setup_s": round(setup_s, 2),
-                   "bytes_alg_per_site": b_alg, **{k: round(v, 6) for k, v in extra.items()}},
+                   "bytes_alg_per_site": b_alg,
+                   **{k: round(v, 6) for k, v in work.meta.items()}},
         "roofline": roof,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -77,6 +77,7 @@ _SIGS = {
     "hlb_get_distributions": (C.c_int, [_P, _P]),
     "hlb_set_distributions": (C.c_int, [_P, _P]),
     "hlb_get_macroscopic": (C.c_int, [_P, _P, _P]),
+    "hlb_get_wall_shear_stress": (C.c_int, [_P, _P]),
     "hlb_reduce": (C.c_int, [_P, C.POINTER(hlb_monitor)]),
     "hlb_snapshot_velocity": (C.c_int, [_P]),
     "hlb_velocity_change": (C.c_int, [_P, C.c_double, C.POINTER(C.c_double)]),

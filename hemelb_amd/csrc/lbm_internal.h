@@ -90,6 +90,11 @@ int launch_import(const DevDomain& d, const StepArgs& a, const double* aos,
 // Moments of a binary64 SoA state: rho[np], u[3][np] (internal order).
 int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
                    double* u, void* stream);
+// Wall shear stress magnitude at nw wall sites (compact index site[i],
+// normal[3*i..]) of a binary64 SoA state; temp = -sqrt(2) * stress parameter.
+int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
+                             const int64_t* site, const float* normal, double temp,
+                             double* out, void* stream);
 // Fixed-order binary64 reductions over moments; partial results per block.
 int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
                   const double* rho, const double* u, double* partials,

@@ -777,6 +777,60 @@ __global__ __launch_bounds__(256) void k_velocity_change(int64_t n, int64_t np,
   if (t == 0) partials[blockIdx.x] = s[0];
 }
 
+// Wall shear stress magnitude at the wall sites (Lattice.h:652-727) from the
+// post-stream binary64 distributions: f_neq = f - f_eq with the SSE3-order
+// equilibrium; Pi_ab = sum_l f_neq_l c_la c_lb accumulated in direction
+// order (terms with c_la c_lb = 0 add nothing); t_a = sum_b (Pi_ab n_b) temp;
+// result sqrt(|t|^2 - (t . n)^2).
+template <int Q, int A, int B>
+__device__ __forceinline__ double second_moment(const double (&fneq)[Q]) {
+  using L = Lat<Q>;
+  double acc = 0.0;
+  sfor<0, Q>([&](auto li) {
+    constexpr int l = decltype(li)::value;
+    constexpr int c = L::c[l][A] * L::c[l][B];
+    if constexpr (c == 1) acc = acc + fneq[l];
+    else if constexpr (c == -1) acc = acc - fneq[l];
+  });
+  return acc;
+}
+
+template <int Q>
+__global__ __launch_bounds__(256) void k_wall_shear_stress(
+    int64_t nw, int64_t np, const double* __restrict__ soa,
+    const int64_t* __restrict__ site, const float* __restrict__ nrm,
+    double temp, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  const int64_t x = site[i];
+  double f[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    f[d] = soa[(int64_t)d * np + x];
+  });
+  double r, mx, my, mz;
+  moments<Q, double>(f, r, mx, my, mz);
+  const FeqCoef<double> fe(r, mx, my, mz);
+  double fneq[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    fneq[d] = f[d] - fe.template eq<Q, d>();
+  });
+  const double p00 = second_moment<Q, 0, 0>(fneq), p10 = second_moment<Q, 1, 0>(fneq),
+               p11 = second_moment<Q, 1, 1>(fneq), p20 = second_moment<Q, 2, 0>(fneq),
+               p21 = second_moment<Q, 2, 1>(fneq), p22 = second_moment<Q, 2, 2>(fneq);
+  const double pi[3][3] = {{p00, p10, p20}, {p10, p11, p21}, {p20, p21, p22}};
+  const double n[3] = {(double)nrm[3 * i], (double)nrm[3 * i + 1], (double)nrm[3 * i + 2]};
+  double sq = 0.0, nn = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    double t = 0.0;
+    for (int b = 0; b < 3; ++b) t += pi[a][b] * n[b] * temp;
+    sq += t * t;
+    nn += t * n[a];
+  }
+  out[i] = sqrt(sq - nn * nn);
+}
+
 template <typename S>
 __global__ void k_gather_pairs(int64_t count, int64_t np, const int64_t* slot,
                                const S* state, double rho0, const double* w,
@@ -894,6 +948,19 @@ int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
   else
     hipLaunchKernelGGL(k_moments<19>, grid_for(n), dim3(256), 0,
                        (hipStream_t)stream, n, np, soa, rho, u);
+  return (int)hipGetLastError();
+}
+
+int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
+                             const int64_t* site, const float* normal, double temp,
+                             double* out, void* stream) {
+  if (nw == 0) return 0;
+  if (Q == 15)
+    hipLaunchKernelGGL(k_wall_shear_stress<15>, grid_for(nw), dim3(256), 0,
+                       (hipStream_t)stream, nw, np, soa, site, normal, temp, out);
+  else
+    hipLaunchKernelGGL(k_wall_shear_stress<19>, grid_for(nw), dim3(256), 0,
+                       (hipStream_t)stream, nw, np, soa, site, normal, temp, out);
   return (int)hipGetLastError();
 }
 

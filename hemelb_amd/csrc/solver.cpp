@@ -11,6 +11,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -250,6 +251,14 @@ struct hlb_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hlb::DevDomain dom;
   hlb::StepArgs args;
+  // wall sites (internal compact index) and their normals, for the wall
+  // shear stress output
+  bool has_normals = false;
+  std::vector<int64_t> wall_sites;
+  std::vector<float> wall_normals;
+  DevBuf<int64_t> d_wall_sites;
+  DevBuf<float> d_wall_normals;
+  DevBuf<double> d_wss;
   std::vector<int64_t> io_sites_internal;  // per iolet-site record
   std::vector<int32_t> io_sites_iolet;
   uint64_t t_ref = 1;
@@ -491,6 +500,14 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
       return fail(HLB_E_INVALID, "two sites share the same coordinates");
     s->perm[o] = keys[o].second;
     s->internal[keys[o].second] = o;
+  }
+  s->has_normals = dm->wall_normal != nullptr;
+  for (int64_t o = 0; o < n; ++o) {
+    const int64_t i = s->perm[o];
+    if (!dm->wall_mask[i]) continue;
+    s->wall_sites.push_back(o);
+    if (s->has_normals)
+      for (int a = 0; a < 3; ++a) s->wall_normals.push_back(dm->wall_normal[3 * i + a]);
   }
   // Site lookup by coordinates: block key -> internal index of slot 0..511.
   std::unordered_map<uint64_t, std::vector<int32_t>> blocks;
@@ -936,6 +953,37 @@ int hlb_get_macroscopic(hlb_solver* s, double* density, double* velocity) {
     if (velocity)
       for (int a = 0; a < 3; ++a) velocity[3 * c + a] = u[(size_t)a * s->npc + o];
   }
+  return HLB_OK;
+}
+
+int hlb_get_wall_shear_stress(hlb_solver* s, double* stress) {
+  if (!s || !stress) return fail(HLB_E_INVALID, "null argument");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  if (!s->has_normals)
+    return fail(HLB_E_STATE, "wall normals were not given to hlb_create");
+  int r = s->post_stream_soa();
+  if (r) return r;
+  const int64_t nw = (int64_t)s->wall_sites.size();
+  std::vector<double> w(nw);
+  if (nw) {
+    if (!s->d_wall_sites.p) {
+      HIP_OR_FAIL(s->d_wall_sites.upload(s->wall_sites));
+      HIP_OR_FAIL(s->d_wall_normals.upload(s->wall_normals));
+      HIP_OR_FAIL(s->d_wss.alloc((size_t)nw));
+    }
+    // LbmParameters::CalcDerivedParams (LbmParameters.h:34-39) and the
+    // factor of CalculateWallShearStressMagnitude (Lattice.h:669).
+    const double stress_parameter = (1.0 - 1.0 / (2.0 * s->p.tau)) / std::sqrt(2.0);
+    const double temp = stress_parameter * (-std::sqrt(2.0));
+    if (hlb::launch_wall_shear_stress(s->Q, nw, s->npc, s->s_soa.p, s->d_wall_sites.p,
+                                      s->d_wall_normals.p, temp, s->d_wss.p, s->stream))
+      return fail(HLB_E_DEVICE, "wall shear stress kernel launch failed");
+    HIP_OR_FAIL(hipMemcpyAsync(w.data(), s->d_wss.p, sizeof(double) * nw,
+                               hipMemcpyDeviceToHost, s->stream));
+  }
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  std::fill(stress, stress + s->n, std::numeric_limits<double>::max());
+  for (int64_t k = 0; k < nw; ++k) stress[s->perm[s->wall_sites[k]]] = w[k];
   return HLB_OK;
 }
 

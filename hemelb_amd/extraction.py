@@ -1,0 +1,113 @@
+"""Property extraction and the periodic-state monitor.
+
+The fields are those the reference's property output writes
+(Code/extraction/LbDataSourceIterator.cc:33-47): pressure = density * Cs2,
+velocity = momentum / density, wall shear stress magnitude at wall sites
+(Code/lb/streamers/Common.h:37-50), all from fOld in binary64 and converted to
+physical units with util::UnitConverter when asked.
+
+The periodic-state monitor generalises StabilityTester's steady-state check
+(Code/lb/StabilityTester.h:160-183) to pulsatile flow: the velocity field is
+sampled at the same phase of consecutive cycles and the run is periodic once
+max_sites |u(t) - u(t - T)| / u_ref drops below the tolerance.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .lb import LBM, Cs2, UnitConverter
+
+NO_VALUE = np.finfo(np.float64).max   # constants.h:47
+
+
+@dataclass
+class Fields:
+    density: np.ndarray            # [n]
+    velocity: np.ndarray           # [n, 3]
+    pressure: np.ndarray           # [n], lattice units (density * Cs2)
+    wall_shear_stress: np.ndarray  # [n], NO_VALUE at sites without wall links
+    wall: np.ndarray               # [n] bool, sites with wall links
+
+    def physical(self, units: UnitConverter) -> "Fields":
+        wss = np.where(self.wall, units.ConvertStressToPhysicalUnits(
+            np.where(self.wall, self.wall_shear_stress, 0.0)), NO_VALUE)
+        return Fields(self.density, units.ConvertVelocityToPhysicalUnits(self.velocity),
+                      units.ConvertPressureToPhysicalUnits(self.pressure), wss, self.wall)
+
+
+def extract(lbm: LBM, wall_shear_stress: bool = True) -> Fields:
+    rho, u = lbm.macroscopic()
+    wall = lbm.domain.wall_mask != 0
+    wss = lbm.wall_shear_stress() if wall_shear_stress else np.full(rho.shape, NO_VALUE)
+    return Fields(rho, u, rho * Cs2, wss, wall)
+
+
+def _norms(diff: np.ndarray, ref: np.ndarray) -> dict:
+    """L2 and Linf of `diff`, absolute and relative to the reference field.
+    Vector fields ([n, 3]) use the per-site Euclidean norm."""
+    if diff.ndim == 2:
+        d, r = np.sqrt((diff ** 2).sum(1)), np.sqrt((ref ** 2).sum(1))
+    else:
+        d, r = np.abs(diff), np.abs(ref)
+    l2, l2r = float(np.sqrt((d ** 2).sum())), float(np.sqrt((r ** 2).sum()))
+    li, lir = float(d.max(initial=0.0)), float(r.max(initial=0.0))
+    return {"l2": l2 / max(np.sqrt(d.size), 1.0), "linf": li,
+            "l2_rel": l2 / l2r if l2r > 0 else float("nan"),
+            "linf_rel": li / lir if lir > 0 else float("nan")}
+
+
+def field_errors(test: Fields, ref: Fields) -> dict:
+    """Errors of `test` against `ref`.
+
+    velocity: per-site vector error, relative to the reference's L2 / max |u|;
+    pressure: relative to the reference's pressure range (max - min), since
+    the absolute pressure (~Cs2) would hide every error;
+    wall_shear_stress: over the wall sites, relative like velocity.
+    `l2` entries are root-mean-square values."""
+    out = {"velocity": _norms(test.velocity - ref.velocity, ref.velocity)}
+    dp = test.pressure - ref.pressure
+    p = _norms(dp, ref.pressure)
+    span = float(ref.pressure.max() - ref.pressure.min())
+    p["l2_rel"] = p["l2"] / span if span > 0 else float("nan")
+    p["linf_rel"] = p["linf"] / span if span > 0 else float("nan")
+    out["pressure"] = p
+    w = ref.wall
+    if w.any() and (ref.wall_shear_stress[w] != NO_VALUE).all():
+        out["wall_shear_stress"] = _norms(test.wall_shear_stress[w] - ref.wall_shear_stress[w],
+                                          ref.wall_shear_stress[w])
+    return out
+
+
+@dataclass
+class PeriodicStateMonitor:
+    """Runs an LBM cycle by cycle until the velocity at the same phase of two
+    consecutive cycles differs by less than `tolerance` (relative to
+    `reference_speed`), or `max_cycles` have run."""
+    lbm: LBM
+    period: float                  # steps per cycle
+    reference_speed: float
+    tolerance: float = 1e-4
+    history: list = field(default_factory=list)
+
+    def run(self, max_cycles: int) -> dict:
+        t0 = time.perf_counter()
+        start = self.lbm.GetTimeStep()
+        self.lbm.snapshot_velocity()
+        converged = False
+        for k in range(1, max_cycles + 1):
+            target = start + int(round(k * self.period))
+            self.lbm.step(target - self.lbm.GetTimeStep())
+            change = self.lbm.velocity_change(self.reference_speed)
+            self.history.append(change)
+            self.lbm.snapshot_velocity()
+            if change < self.tolerance:
+                converged = True
+                break
+        self.lbm.synchronize()
+        return {"converged": converged, "cycles": len(self.history),
+                "steps": self.lbm.GetTimeStep() - start,
+                "seconds": time.perf_counter() - t0,
+                "last_change": self.history[-1] if self.history else float("nan")}

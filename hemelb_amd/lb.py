@@ -149,6 +149,22 @@ class UnitConverter:
     def ConvertSpeedToLatticeUnits(self, v):
         return v / (self.dx / self.dt)
 
+    # Output conversions (UnitConverter.cc:31-46, UnitConverter.h:58-120),
+    # vectorised over numpy arrays.
+    @property
+    def lattice_pressure(self):
+        lattice_mass = self.rho * self.dx * self.dx * self.dx
+        return lattice_mass / (self.dx * self.dt * self.dt)
+
+    def ConvertPressureToPhysicalUnits(self, p_lattice):
+        return self.ref + (p_lattice - Cs2) * self.lattice_pressure / mmHg_TO_PASCAL
+
+    def ConvertVelocityToPhysicalUnits(self, u):
+        return u * (self.dx / self.dt)
+
+    def ConvertStressToPhysicalUnits(self, s):
+        return s * self.lattice_pressure
+
 
 # ---------------------------------------------------------------------------
 # Iolets (Code/lb/iolets)
@@ -307,6 +323,13 @@ class LBM:
         N.check(N.lib().hlb_get_macroscopic(self._h, rho.ctypes.data_as(C.c_void_p),
                                             u.ctypes.data_as(C.c_void_p)))
         return rho, u
+
+    def wall_shear_stress(self) -> np.ndarray:
+        """Per-site wall shear stress magnitude of fOld (lattice units);
+        float max (the reference's NO_VALUE) at sites without wall links."""
+        out = np.empty(self.domain.n, np.float64)
+        N.check(N.lib().hlb_get_wall_shear_stress(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def monitor(self) -> dict:
         m = N.hlb_monitor()

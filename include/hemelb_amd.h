@@ -170,6 +170,15 @@ int hlb_set_distributions(hlb_solver* s, const double* f_in);
  * be NULL), as MacroscopicPropertyCache / LbDataSourceIterator compute them. */
 int hlb_get_macroscopic(hlb_solver* s, double* density, double* velocity);
 
+/* Wall shear stress magnitude [n] of fOld in the caller's order: at sites
+ * with wall links, Lattice::CalculateWallShearStressMagnitude
+ * (Lattice.h:652-689) with the stress parameter of LbmParameters
+ * (LbmParameters.h:34-39), as UpdateCachePostCollision fills
+ * wallShearStressMagnitudeCache (Code/lb/streamers/Common.h:37-50); DBL_MAX
+ * (the reference's NO_VALUE, constants.h:47) elsewhere.  Needs
+ * hlb_domain.wall_normal at create. */
+int hlb_get_wall_shear_stress(hlb_solver* s, double* stress);
+
 int hlb_reduce(hlb_solver* s, hlb_monitor* out);
 
 /* Convergence monitor: hlb_snapshot_velocity stores the current velocity

@@ -63,6 +63,8 @@ _SIGS = {
     "orc_get_f": (None, [_V, _V]),
     "orc_set_f": (None, [_V, _V]),
     "orc_macro": (None, [_V, _V, _V]),
+    "orc_wall_shear_stress": (_D, [C.c_int, _D, _V, _V]),
+    "orc_wall_shear_stress_all": (None, [_V, _V, _V]),
 }
 _REF_SIGS = {
     "ref_lattice": (C.c_int, [C.c_int, _V, _V, _V, _V, _V]),
@@ -72,6 +74,7 @@ _REF_SIGS = {
     "ref_bessel_j0": (None, [_D, _D, _D, _V]),
     "ref_ipow_three_halves": (None, [_V]),
     "ref_pressure_to_lattice": (_D, [_D] * 5),
+    "ref_wall_shear_stress": (_D, [C.c_int, _D, _V, _V]),
     "ref_pressure_diff_to_lattice": (_D, [_D] * 4),
     "ref_pressure_gradient_to_lattice": (_D, [_D] * 4),
 }
@@ -174,3 +177,10 @@ class OracleSim:
         u = np.empty((self.n, 3))
         lib().orc_macro(self._h, _p(rho), _p(u))
         return rho, u
+
+    def wall_shear_stress(self, normal):
+        """Per-site WSS magnitude of fOld; float max at non-wall sites."""
+        nrm = np.ascontiguousarray(normal, np.float32)
+        out = np.empty(self.n)
+        lib().orc_wall_shear_stress_all(self._h, _p(nrm), _p(out))
+        return out

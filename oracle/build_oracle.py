@@ -2,7 +2,7 @@
 
 * ``oracle/liboracle.so`` -- the C restatement in ``hlb_oracle.c``; always built.
 * ``oracle/_ref/libhlbref.so`` -- ``ref_glue.cpp`` compiled together with the
-  reference's own ``Code/util/Bessel.cc``, ``Code/util/UnitConverter.cc`` and
+  reference's own ``Code/util/Bessel.cc``, ``Code/util/UnitConverter.cc``, ``Code/util/Matrix3D.cc`` and
   ``Code/lb/kernels/DHumieresD3Q{15,19}MRTBasis.cc`` where they lie under
   ``/root/reference`` (same flags as the reference's default x86_64 build:
   ``-std=c++20 -msse3 -DHEMELB_USE_SSE3``).  Built only when that tree exists;
@@ -41,7 +41,7 @@ def build_ref() -> str | None:
     out = os.path.join(outdir, "libhlbref.so")
     srcs = [os.path.join(HERE, "ref_glue.cpp")] + [
         os.path.join(REF_CODE, p) for p in (
-            "util/Bessel.cc", "util/UnitConverter.cc",
+            "util/Bessel.cc", "util/UnitConverter.cc", "util/Matrix3D.cc",
             "lb/kernels/DHumieresD3Q15MRTBasis.cc",
             "lb/kernels/DHumieresD3Q19MRTBasis.cc")]
     _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-msse3",

@@ -14,6 +14,7 @@
 #include "hlb_oracle.h"
 
 #include <complex.h>
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -157,6 +158,41 @@ void orc_feq(int Q, double rho, const double* m, double* feq) {
   lattice_t L;
   if (lattice_init(Q, &L)) return;
   feq_all(&L, rho, m, feq);
+}
+
+/* Wall shear stress magnitude of one site (Lattice.h:652-689), from
+ * f_neq = f - f_eq (LBGK.h:140-150; MRT.h:55-66) and the stress parameter
+ * (1 - 1/(2 tau))/sqrt(2) (LbmParameters.h:34-39).  The second moment
+ * Pi_ij = sum_l f_neq_l c_li c_lj is accumulated over l in order
+ * (Lattice.h:700-727); the traction t_i = sum_j (Pi_ij n_j) temp with
+ * temp = stress_parameter * -sqrt(2); result sqrt(|t|^2 - (t.n)^2). */
+double orc_wall_shear_stress(int Q, double tau, const double* f,
+                             const float* normal) {
+  lattice_t L;
+  if (lattice_init(Q, &L)) return -1.0;
+  double rho, m[3], feq[19], fneq[19];
+  dens_mom(&L, f, &rho, m);
+  feq_all(&L, rho, m, feq);
+  for (int i = 0; i < Q; ++i) fneq[i] = f[i] - feq[i];
+  const double stress_parameter = (1.0 - 1.0 / (2.0 * tau)) / sqrt(2.0);
+  const double temp = stress_parameter * (-sqrt(2.0));
+  double pi[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b <= a; ++b) {
+      double acc = 0.0;
+      for (int l = 0; l < Q; ++l)
+        acc += fneq[l] * L.cd[a][l] * L.cd[b][l];
+      pi[a][b] = acc;
+      pi[b][a] = acc;
+    }
+  const double n[3] = {(double)normal[0], (double)normal[1], (double)normal[2]};
+  double t[3] = {0.0, 0.0, 0.0}, sq = 0.0, nn = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    for (int b = 0; b < 3; ++b) t[a] += pi[a][b] * n[b] * temp;
+    sq += t[a] * t[a];
+    nn += t[a] * n[a];
+  }
+  return sqrt(sq - nn * nn);
 }
 
 /* ------------------------------------------------------------------ */
@@ -731,4 +767,13 @@ void orc_macro(const orc_sim* s, double* rho, double* u) {
     if (u)
       for (int a = 0; a < 3; ++a) u[3 * i + a] = m[a] / r;
   }
+}
+
+void orc_wall_shear_stress_all(const orc_sim* s, const float* normal,
+                               double* out) {
+  const int Q = s->L.Q;
+  for (int64_t i = 0; i < s->n; ++i)
+    out[i] = s->wmask[i] ? orc_wall_shear_stress(Q, s->tau, &s->f_old[i * Q],
+                                                     &normal[3 * i])
+                             : DBL_MAX;
 }

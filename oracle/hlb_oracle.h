@@ -64,6 +64,10 @@ void orc_collide_lbgk(int Q, double tau, const double* f, double* fpc,
 void orc_collide_mrt(int Q, double tau, const double* s, const double* f,
                      double* fpc);
 int orc_mrt_basis(int Q, double* M, double* diag, double* s_for_tau, double tau);
+/* Wall shear stress magnitude of a site with distributions f (fOld) and
+ * wall normal `normal` (Lattice.h:652-689). */
+double orc_wall_shear_stress(int Q, double tau, const double* f,
+                             const float* normal);
 void orc_bessel_j0(double re, double im, double tolsq, double* out);
 void orc_ipow_three_halves(double* out);
 double orc_cosine_density(const orc_iolet* io, uint64_t t0);
@@ -94,6 +98,10 @@ int orc_step(orc_sim* s, int64_t n);
 void orc_get_f(const orc_sim* s, double* out);
 void orc_set_f(orc_sim* s, const double* in);
 void orc_macro(const orc_sim* s, double* rho, double* u);
+/* Per-site wall shear stress of fOld; DBL_MAX (NO_VALUE, constants.h:47) at
+ * sites without wall links (Common.h:37-50). */
+void orc_wall_shear_stress_all(const orc_sim* s, const float* normal,
+                               double* out);
 
 #ifdef __cplusplus
 }

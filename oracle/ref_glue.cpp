@@ -2,7 +2,7 @@
 // from their own sources without any external dependency:
 //   Code/lb/lattices/{Lattice,D3Q15,D3Q19}.h   (header only)
 //   Code/lb/kernels/DHumieresD3Q{15,19}MRTBasis.{h,cc}
-//   Code/util/Bessel.cc, Code/util/UnitConverter.cc
+//   Code/util/Bessel.cc, Code/util/UnitConverter.cc, Code/util/Matrix3D.cc
 // TEST INFRASTRUCTURE: built by oracle/build_oracle.py into oracle/_ref/ only
 // when the reference tree is present, and used by tests/ to pin the C oracle
 // bit for bit.  The streamers, HydroVars and kernels need geometry::Domain
@@ -57,6 +57,27 @@ int basis(double* M, double* diag, double* s, double tau) {
   return B::NUMMOMENTS;
 }
 
+// f_neq = f - f_eq (LBGK.h:140-150, MRT.h:55-66), then
+// CalculateWallShearStressMagnitude with LbmParameters' stress parameter
+// (LbmParameters.h:34-39) and the float wall normal widened to double.
+template <class L>
+double wss(double tau, const double* f, const float* nrm) {
+  std::array<distribn_t, L::NUMVECTORS> fa, fe, fneq;
+  std::memcpy(fa.data(), f, sizeof(double) * L::NUMVECTORS);
+  distribn_t density;
+  util::Vector3D<distribn_t> mom, vel;
+  L::CalculateDensityMomentumFEq(typename L::const_span(fa), density, mom, vel,
+                                 typename L::mut_span(fe));
+  for (unsigned i = 0; i < L::NUMVECTORS; ++i) fneq[i] = fa[i] - fe[i];
+  const double stress_parameter = (1.0 - 1.0 / (2.0 * tau)) / std::sqrt(2.0);
+  const util::Vector3D<float> n(nrm[0], nrm[1], nrm[2]);
+  distribn_t stress;
+  L::CalculateWallShearStressMagnitude(density, typename L::const_span(fneq),
+                                       util::Vector3D<double>(n), stress,
+                                       stress_parameter);
+  return stress;
+}
+
 template <class L>
 void lattice(int* cx, int* cy, int* cz, double* w, int* inv) {
   for (unsigned i = 0; i < L::NUMVECTORS; ++i) {
@@ -90,6 +111,12 @@ int ref_feq(int Q, double rho, const double* m, double* feq) {
   else if (Q == 19) fq<lb::D3Q19>(rho, m, feq);
   else return -1;
   return 0;
+}
+
+double ref_wall_shear_stress(int Q, double tau, const double* f, const float* normal) {
+  if (Q == 15) return wss<lb::D3Q15>(tau, f, normal);
+  if (Q == 19) return wss<lb::D3Q19>(tau, f, normal);
+  return -1.0;
 }
 
 int ref_mrt_basis(int Q, double* M, double* diag, double* s, double tau) {

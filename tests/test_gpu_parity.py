@@ -77,6 +77,22 @@ def test_macroscopic_and_monitor_match_oracle():
         np.testing.assert_allclose(m["iolet_flux"][k], expect, rtol=1e-12, atol=1e-15)
 
 
+@pytest.mark.parametrize("label,make", [
+    ("fourcube-mrt-bfl", lambda: K.four_cube_case("MRT", "BFL")),
+    ("cyl19-lbgk-bfl-ladd-wom",
+     lambda: K.cylinder_case(6.0, 24, 19, "LBGK", "BFL", "womersley", tau=0.7)),
+    ("cyl15-lbgk-sbb-nash", lambda: K.cylinder_case(6.0, 24, 15, "LBGK", "SBB", "cosine"))])
+def test_wall_shear_stress_matches_oracle(label, make):
+    case = make()
+    orc, gpu = _run_and_compare(case, 20, label="wss " + label)
+    w_o = orc.wall_shear_stress(case.domain.normal)
+    w_g = gpu.wall_shear_stress()
+    np.testing.assert_array_equal(w_g, w_o)
+    wall = case.domain.wall_mask != 0
+    assert (w_g[~wall] == np.finfo(np.float64).max).all()
+    assert np.isfinite(w_g[wall]).all() and (w_g[wall] > 0).any()
+
+
 def test_export_import_resume_is_exact():
     case = K.cylinder_case(6.0, 24, 19, "MRT", "BFL", "womersley", tau=0.7)
     orc, gpu = _run_and_compare(case, 10, label="resume-a")

@@ -438,3 +438,68 @@ def test_pressure_driven_cylinder_approaches_poiseuille():
     # profile shape: parabolic in r
     ring = mid & (np.abs(np.sqrt(r2) - R / 2) < 0.5)
     assert abs(u[ring, 2].mean() / u_c - 0.75) < 0.06
+
+
+# ---------------------------------------------------------------------------
+# Wall shear stress (Lattice.h:652-727)
+# ---------------------------------------------------------------------------
+def _wss_inputs(Q, k, rng):
+    rho = rng.uniform(0.8, 1.2)
+    feq = np.zeros(Q)
+    O.lib().orc_feq(Q, rho, P(rng.normal(0, 0.03, 3) * rho), P(feq))
+    f = feq * (1 + rng.normal(0, 1e-2, Q))
+    n = rng.normal(size=3)
+    n = (n / np.linalg.norm(n)).astype(np.float32)
+    if k % 5 == 0:
+        n = np.float32([0, 0, 1]) if k % 2 else np.float32([1, 0, 0])
+    return f, n, rng.uniform(0.51, 2.0)
+
+
+@needs_ref
+@pytest.mark.parametrize("Q", [15, 19])
+def test_wall_shear_stress_bitwise_equal_reference(Q):
+    rng = np.random.default_rng(40 + Q)
+    for k in range(2000):
+        f, n, tau = _wss_inputs(Q, k, rng)
+        a = O.lib().orc_wall_shear_stress(Q, tau, P(f), P(n))
+        b = O.ref().ref_wall_shear_stress(Q, tau, P(f), P(n))
+        assert a == b or (math.isnan(a) and math.isnan(b)), (k, a, b)
+
+
+@pytest.mark.parametrize("Q", [15, 19])
+def test_wall_shear_stress_is_the_tangential_traction(Q):
+    """|t - (t.n) n| with t = sigma n and sigma = -(1 - 1/(2 tau)) Pi_neq
+    (the deviatoric stress of Chen & Doolen 1998), computed in numpy.  The
+    reference's sqrt(|t|^2 - (t.n)^2) assumes |n| = 1, which a float normal
+    meets to ~1e-8: tight tolerance for axis normals, 1e-6 otherwise."""
+    c, w, _ = _lattice(Q)
+    rng = np.random.default_rng(7)
+    for k in range(300):
+        f, n, tau = _wss_inputs(Q, k, rng)
+        rho, m = np.zeros(1), np.zeros(3)
+        O.lib().orc_density_momentum(Q, P(f), P(rho), P(m))
+        fneq = f - _textbook_feq(Q, rho[0], m)
+        sigma = -(1 - 1 / (2 * tau)) * np.einsum("l,la,lb->ab", fneq, c, c)
+        nd = n.astype(np.float64)
+        t = sigma @ nd
+        want = np.linalg.norm(t - (t @ nd) * nd)
+        got = O.lib().orc_wall_shear_stress(Q, tau, P(f), P(n))
+        tol = 1e-12 if k % 5 == 0 else 1e-6
+        assert abs(got - want) <= tol * np.linalg.norm(t) + 1e-18, (k, got, want)
+
+
+@pytest.mark.parametrize("Q", [15, 19])
+def test_wall_shear_stress_of_a_simple_shear_flow(Q):
+    """Chapman-Enskog non-equilibrium of u_x = gamma*y:
+    f_neq_i = -(tau w_i rho / cs^2) c_ix c_iy gamma, whose wall shear stress
+    on a wall with normal y is the viscous stress rho*nu*gamma,
+    nu = cs^2 (tau - 1/2)."""
+    c, w, _ = _lattice(Q)
+    for tau, rho, gamma in [(0.8, 1.0, 1e-3), (0.55, 51.0, 3e-4), (1.7, 0.97, -2e-3)]:
+        feq = np.zeros(Q)
+        m = np.array([0.01, 0.0, 0.0]) * rho
+        O.lib().orc_feq(Q, rho, P(m), P(feq))
+        f = feq - tau * w * rho * 3.0 * c[:, 0] * c[:, 1] * gamma
+        got = O.lib().orc_wall_shear_stress(Q, tau, P(f), P(np.float32([0, 1, 0])))
+        want = rho * (tau - 0.5) / 3.0 * abs(gamma)
+        assert abs(got - want) <= 1e-12 * want
