@@ -56,6 +56,14 @@ class hlb_monitor(C.Structure):
                 ("iolet_flux", C.c_double * 64)]
 
 
+HLBX_FLOAT, HLBX_DOUBLE, HLBX_INT32, HLBX_UINT32, HLBX_INT64, HLBX_UINT64 = range(6)
+
+
+class hlbx_field(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_int32), ("n_elements", C.c_int32),
+                ("n_offsets", C.c_int32), ("offsets", C.POINTER(C.c_double))]
+
+
 class hlbg_iolet(C.Structure):
     _fields_ = [("position", C.c_double * 3), ("normal", C.c_double * 3),
                 ("radius", C.c_double)]
@@ -110,6 +118,14 @@ _SIGS = {
     "hlbg_get_links": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "hlbg_project": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
     "hlbg_last_error": (C.c_char_p, []),
+    # extraction files
+    "hlbx_create": (C.c_int, [C.c_char_p, C.c_double, _P, C.c_int64, _P, C.c_int32,
+                              C.POINTER(hlbx_field), C.POINTER(_P)]),
+    "hlbx_append": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "hlbx_close": (C.c_int, [_P]),
+    "hlbx_read_distributions": (C.c_int, [C.c_char_p, C.c_int64, _P, C.c_int32, C.c_int64,
+                                          _P, C.POINTER(C.c_uint64)]),
+    "hlbx_last_error": (C.c_char_p, []),
 }
 
 _lib = None
@@ -138,9 +154,10 @@ class HemeLBError(RuntimeError):
         self.code = code
 
 
-def check(code: int, geometry: bool = False) -> None:
+def check(code: int, geometry: bool = False, io: bool = False) -> None:
     if code != HLB_OK:
-        fn = lib().hlbg_last_error if geometry else lib().hlb_last_error
+        fn = (lib().hlbx_last_error if io else
+              lib().hlbg_last_error if geometry else lib().hlb_last_error)
         raise HemeLBError(code, (fn() or b"").decode())
 
 
@@ -151,6 +168,6 @@ def declared_functions(header_paths) -> list[str]:
     for p in header_paths:
         txt = open(p).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(hlbg?_\w+)\s*\(", txt, flags=re.M):
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(hlb[gx]?_\w+)\s*\(", txt, flags=re.M):
             names.append(m.group(1))
     return sorted(set(names))

@@ -2,8 +2,9 @@
 
 * ``csrc/lbm_kernels.hip`` -> hipcc for gfx950 (device code; no FMA
   contraction so binary64 results match the reference's evaluation order).
-* ``csrc/solver.cpp`` and ``csrc/geometry.cpp`` -> g++ (host code; the same
-  compiler runtime and libm as the reference's own g++ build).
+* ``csrc/solver.cpp``, ``csrc/geometry.cpp`` and ``csrc/extraction.cpp`` ->
+  g++ (host code; the same compiler runtime and libm as the reference's own
+  g++ build).
 * linked with g++ against libamdhip64 and zlib.
 
 Usage: ``python -m hemelb_amd.build`` (or ``__graft_entry__.build()``).
@@ -54,7 +55,7 @@ def build() -> str:
     _run([os.path.join(ROCM, "bin", "hipcc")] + HIPCC_FLAGS +
          ["-c", os.path.join(CSRC, "lbm_kernels.hip"), "-o", k_obj])
     objs.append(k_obj)
-    for src in ("solver.cpp", "geometry.cpp"):
+    for src in ("solver.cpp", "geometry.cpp", "extraction.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++"] + GXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -262,6 +262,31 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
       constexpr int d = decltype(di)::value;
       fneq[d] = f[d] - fe.template eq<Q, d>();
     });
+    if constexpr (KERNEL == 0) {
+      // In binary32 the rounded weights give sum_i w_i c_i c_i = Cs2 (1 + 7e-9)
+      // I, so the equilibrium carries momentum m (1 + 7e-9): relaxing every
+      // direction towards it (LBGK) would add a body force of ~1e-8 m per
+      // step, a 1e-5 relative velocity bias in slow vessels.  Remove the mass
+      // and momentum of f_neq so that the collision conserves both.  (MRT
+      // never relaxes the conserved moments.)
+      using L = Lat<Q>;
+      T s0 = T(0), sx = T(0), sy = T(0), sz = T(0);
+      sfor<0, Q>([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        s0 = s0 + fneq[d];
+        sx = addc<L::c[d][0]>(sx, fneq[d]);
+        sy = addc<L::c[d][1]>(sy, fneq[d]);
+        sz = addc<L::c[d][2]>(sz, fneq[d]);
+      });
+      sx = T(3) * sx;
+      sy = T(3) * sy;
+      sz = T(3) * sz;
+      sfor<0, Q>([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        const T e = dotc<L::c[d][0], L::c[d][1], L::c[d][2]>(sx, sy, sz);
+        fneq[d] = fneq[d] - T(L::w[d]) * (s0 + e);
+      });
+    }
   } else {
     h.rho = r;
     FeqCoef<T> fe(r, h.mx, h.my, h.mz);

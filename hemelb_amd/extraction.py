@@ -111,3 +111,126 @@ class PeriodicStateMonitor:
                 "steps": self.lbm.GetTimeStep() - start,
                 "seconds": time.perf_counter() - t0,
                 "last_change": self.history[-1] if self.history else float("nan")}
+
+
+# ---------------------------------------------------------------------------
+# Extraction (.xtr) files and checkpoints
+# ---------------------------------------------------------------------------
+TYPES = {"float": 0, "double": 1, "int32": 2, "uint32": 3, "int64": 4, "uint64": 5}
+
+
+@dataclass
+class OutputField:
+    """One field of an extraction file (extraction::OutputField)."""
+    name: str
+    type: str = "float"
+    n_elements: int = 1
+    offsets: tuple = ()
+
+
+class ExtractionWriter:
+    """An .xtr v5 file and its .off offset file for one rank
+    (extraction::LocalPropertyOutput, Code/extraction/LocalPropertyOutput.cc).
+    `append` writes one record: the time step then, per site, its coordinates
+    and each field's values."""
+
+    def __init__(self, path: str, voxel_size_m: float, origin_m, coords: np.ndarray,
+                 fields: list):
+        import ctypes as C
+
+        from . import _native as N
+        self._N = N
+        self.coords = np.ascontiguousarray(coords, np.int32)
+        self.fields = list(fields)
+        self._offs = [np.ascontiguousarray(f.offsets, np.float64) for f in self.fields]
+        specs = (N.hlbx_field * max(1, len(self.fields)))()
+        for k, f in enumerate(self.fields):
+            specs[k].name = f.name.encode()
+            specs[k].type = TYPES[f.type]
+            specs[k].n_elements = int(f.n_elements)
+            specs[k].n_offsets = len(f.offsets)
+            specs[k].offsets = self._offs[k].ctypes.data_as(C.POINTER(C.c_double))
+        origin = np.ascontiguousarray(origin_m, np.float64)
+        h = C.c_void_p()
+        N.check(N.lib().hlbx_create(path.encode(), float(voxel_size_m),
+                                    origin.ctypes.data_as(C.c_void_p), self.coords.shape[0],
+                                    self.coords.ctypes.data_as(C.c_void_p), len(self.fields),
+                                    specs, C.byref(h)), io=True)
+        self._h = h
+
+    def append(self, time_step: int, arrays: list) -> None:
+        import ctypes as C
+        n = self.coords.shape[0]
+        data = [np.ascontiguousarray(a, np.float64).reshape(n, f.n_elements)
+                for a, f in zip(arrays, self.fields)]
+        ptrs = (C.c_void_p * max(1, len(data)))(*[d.ctypes.data for d in data])
+        self._N.check(self._N.lib().hlbx_append(self._h, int(time_step), ptrs), io=True)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._N.check(self._N.lib().hlbx_close(self._h), io=True)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def checkpoint_writer(lbm: LBM, path: str, voxel_size_m: float = 1.0,
+                      origin_m=(0.0, 0.0, 0.0)) -> ExtractionWriter:
+    """A distribution checkpoint file: one binary64 field of Q values per
+    site named "distributions" (SimConfig.cc:304-326)."""
+    return ExtractionWriter(path, voxel_size_m, origin_m, lbm.domain.coords,
+                            [OutputField("distributions", "double", lbm.Q)])
+
+
+def write_checkpoint(writer: ExtractionWriter, lbm: LBM) -> int:
+    """Appends the record of the current time step: fOld of the step about to
+    run, as PropertyActor::EndIteration writes it (PropertyActor.cc:77-82)."""
+    t = lbm.GetTimeStep()
+    writer.append(t, [lbm.get_distributions()])
+    return t
+
+
+def load_checkpoint(lbm: LBM, path: str, time_step: int | None = None) -> int:
+    """CheckpointInitialCondition (InitialCondition.cc:15-22,
+    LocalDistributionInput.cc:38-170): the record of `time_step` (default:
+    the last one) becomes fOld and the simulation time."""
+    import ctypes as C
+
+    from . import _native as N
+    f = np.empty((lbm.domain.n, lbm.Q), np.float64)
+    t = C.c_uint64()
+    N.check(N.lib().hlbx_read_distributions(
+        path.encode(), lbm.domain.n, lbm.domain.coords.ctypes.data_as(C.c_void_p), lbm.Q,
+        -1 if time_step is None else int(time_step), f.ctypes.data_as(C.c_void_p),
+        C.byref(t)), io=True)
+    lbm.set_distributions(f)
+    lbm.SetTimeStep(int(t.value))
+    return int(t.value)
+
+
+def property_writer(lbm: LBM, path: str, units: UnitConverter, origin_m=(0.0, 0.0, 0.0),
+                    reference_pressure_mmHg: float = 0.0) -> ExtractionWriter:
+    """Pressure (mmHg, stored as float minus the reference pressure), velocity
+    (m/s) and wall shear stress (Pa) as float fields, the reference's default
+    property output types (SimConfig.cc:457)."""
+    return ExtractionWriter(path, units.dx, origin_m, lbm.domain.coords, [
+        OutputField("pressure", "float", 1, (reference_pressure_mmHg,)),
+        OutputField("velocity", "float", 3),
+        OutputField("shearstress", "float", 1)])
+
+
+def write_properties(writer: ExtractionWriter, lbm: LBM, units: UnitConverter) -> int:
+    f = extract(lbm).physical(units)
+    t = lbm.GetTimeStep()
+    writer.append(t, [f.pressure, f.velocity, f.wall_shear_stress])
+    return t

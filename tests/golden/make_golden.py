@@ -9,6 +9,11 @@ own tests hold (run once in a container that has the reference tree):
   programmatic FourCube domain.
 * mrt_basis_diag.json -- BASIS_TIMES_BASIS_TRANSPOSED expected by
   Code/tests/lb/KernelTests.cc:358-370.
+* xtr_headers.json -- the main and field header bytes that
+  Code/tests/extraction/LocalPropertyOutputTests.cc:131-190 expects for the
+  dummy data source of Code/tests/extraction/DummyDataSource.h:22-40
+  (voxel 0.3e-3 m, origin (0.034, 0.001, 0.074), 64 sites; a float Pressure
+  field with one offset, 8 mmHg, and a float Velocity field).
 """
 import json
 import os
@@ -40,6 +45,34 @@ def main():
                "19": [float(v) for v in d19.split(",")],
                "source": "Code/tests/lb/KernelTests.cc:358-370"},
               open(os.path.join(HERE, "mrt_basis_diag.json"), "w"), indent=1)
+
+    lp = open(os.path.join(REF, "extraction", "LocalPropertyOutputTests.cc")).read()
+    json.dump({"main_header": _c_bytes(lp, "expectedMainHeader[] =").hex(),
+               "field_header": _c_bytes(lp, "expectedFieldHeader[] =").hex(),
+               "voxel_size": 0.3e-3, "origin": [0.034, 0.001, 0.074], "sites": 64,
+               "reference_pressure_mmHg": 8.0,
+               "source": "Code/tests/extraction/LocalPropertyOutputTests.cc:131-190"},
+              open(os.path.join(HERE, "xtr_headers.json"), "w"), indent=1)
+
+
+def _c_bytes(src: str, marker: str) -> bytes:
+    """Bytes of the concatenated C string literals after `marker`."""
+    body = src[src.index(marker) + len(marker):]
+    body = body[:body.index(";")]
+    out = bytearray()
+    for lit in re.findall(r'"((?:[^"\\]|\\.)*)"', body):
+        i = 0
+        while i < len(lit):
+            if lit[i] == "\\" and lit[i + 1] == "x":
+                j = i + 2
+                while j < len(lit) and lit[j] in "0123456789abcdefABCDEF":
+                    j += 1
+                out.append(int(lit[i + 2:j], 16))
+                i = j
+            else:
+                out.append(ord(lit[i]))
+                i += 1
+    return bytes(out)
 
 
 if __name__ == "__main__":

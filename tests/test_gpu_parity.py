@@ -106,6 +106,23 @@ def test_export_import_resume_is_exact():
     np.testing.assert_array_equal(gpu2.get_distributions(), orc.get_f())
 
 
+def test_checkpoint_file_resume_is_exact(tmp_path):
+    """Write fOld to an .xtr checkpoint, start a new solver from it
+    (CheckpointInitialCondition) and continue in step with the oracle."""
+    from hemelb_amd import extraction as X
+    case = K.cylinder_case(6.0, 24, 19, "LBGK", "BFL", "womersley", tau=0.7)
+    orc, gpu = _run_and_compare(case, 12, label="checkpoint")
+    path = str(tmp_path / "cp.xtr")
+    w = X.checkpoint_writer(gpu, path, 5e-5)
+    X.write_checkpoint(w, gpu)
+    w.close()
+    gpu2 = case.product()
+    assert X.load_checkpoint(gpu2, path) == gpu.GetTimeStep() == orc.time()
+    orc.step(10)
+    gpu2.step(10)
+    np.testing.assert_array_equal(gpu2.get_distributions(), orc.get_f())
+
+
 def test_instability_is_reported():
     case = K.cylinder_case(5.0, 16, 15, "LBGK", "SBB", "cosine")
     gpu = case.product()
