@@ -20,7 +20,7 @@ HLB_IOLET_NASH, HLB_IOLET_LADD = 0, 1
 HLB_IOLET_COSINE, HLB_IOLET_PARABOLIC, HLB_IOLET_WOMERSLEY = 0, 1, 2
 HLB_SITE_FLUID, HLB_SITE_INLET, HLB_SITE_OUTLET = 1, 2, 3
 HLB_STORE_FP64, HLB_STORE_FP32, HLB_STORE_FP16 = 0, 1, 2
-HLB_LAYOUT_BLOCKS, HLB_LAYOUT_PULL_TABLE = 0, 1
+HLB_LAYOUT_BLOCKS, HLB_LAYOUT_PULL_TABLE, HLB_LAYOUT_COMPACT_BLOCKS = 0, 1, 2
 HLBG_CUT_NONE, HLBG_CUT_WALL, HLBG_CUT_INLET, HLBG_CUT_OUTLET = 0, 1, 2, 3
 
 
@@ -56,6 +56,17 @@ class hlb_monitor(C.Structure):
                 ("iolet_flux", C.c_double * 64)]
 
 
+class hlb_instance_monitor(C.Structure):
+    _fields_ = [("mass", C.c_double), ("density_min", C.c_double),
+                ("density_max", C.c_double), ("velocity_max", C.c_double),
+                ("stable", C.c_int32), ("reserved", C.c_int32), ("sites", C.c_int64)]
+
+
+class hlb_launch_shape(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("planes", C.c_int32), ("groups", C.c_int32),
+                ("xcd_remap", C.c_int32)]
+
+
 HLBX_FLOAT, HLBX_DOUBLE, HLBX_INT32, HLBX_UINT32, HLBX_INT64, HLBX_UINT64 = range(6)
 
 
@@ -87,6 +98,16 @@ _SIGS = {
     "hlb_get_macroscopic": (C.c_int, [_P, _P, _P]),
     "hlb_get_wall_shear_stress": (C.c_int, [_P, _P]),
     "hlb_reduce": (C.c_int, [_P, C.POINTER(hlb_monitor)]),
+    "hlb_create_ensemble": (C.c_int, [C.POINTER(hlb_params), C.POINTER(hlb_domain), _P,
+                                      C.c_int32, _P, C.POINTER(hlb_iolet), C.c_int32,
+                                      C.POINTER(hlb_iolet), C.c_int32, C.POINTER(_P)]),
+    "hlb_instance_count": (C.c_int32, [_P]),
+    "hlb_reduce_instances": (C.c_int, [_P, C.POINTER(hlb_instance_monitor)]),
+    "hlb_velocity_change_instances": (C.c_int, [_P, _P, _P]),
+    "hlb_set_launch_shape": (C.c_int, [_P, C.POINTER(hlb_launch_shape)]),
+    "hlb_get_launch_shape": (C.c_int, [_P, C.POINTER(hlb_launch_shape)]),
+    "hlb_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(hlb_launch_shape),
+                               C.POINTER(C.c_double)]),
     "hlb_snapshot_velocity": (C.c_int, [_P]),
     "hlb_velocity_change": (C.c_int, [_P, C.c_double, C.POINTER(C.c_double)]),
     "hlb_halo_pack": (C.c_int, [_P, _P, C.c_int64, _P]),

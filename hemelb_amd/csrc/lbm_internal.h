@@ -51,6 +51,12 @@ struct DevDomain {
   const float* iolet_nf = nullptr;    // [n_iolets][3] normal rounded to float
   const double* iolet_n = nullptr;    // [n_iolets][3] normal
   const int32_t* iolet_kind = nullptr;  // [n_iolets] hlb_iolet_kind
+  // Per-block relaxation of an ensemble (block layout; null otherwise):
+  // -1/tau, its binary32 rounding and the MRT rate 1/tau of the block's
+  // instance.
+  const double* blk_omega = nullptr;
+  const float* blk_omega_f = nullptr;
+  const double* blk_s_tau = nullptr;
 };
 
 struct StepArgs {
@@ -67,6 +73,18 @@ struct StepArgs {
   // (Nash: ghost density; Ladd Womersley: e^{i w t}; Ladd parabolic: speed).
   const double* bv = nullptr;
   int* unstable = nullptr;  // set to 1 if a post-stream f is not > 0
+
+  // Launch shape of the step kernel in the block layout: threads per
+  // workgroup (64, 128 or 256), x planes per wave, workgroups in the grid
+  // (0: one pass over the domain; fewer: a persistent grid that strides over
+  // it) and whether workgroups are remapped so that each XCD takes one
+  // contiguous range (off by default: measured slower on the vessel tree,
+  // where consecutive workgroups on all XCDs sharing one region of the
+  // domain make better use of the shared last-level cache).
+  uint32_t threads = 256;
+  uint32_t planes = 1;
+  uint32_t groups = 0;
+  uint32_t xcd_remap = 0;
 };
 
 // Launches; every function returns 0 or a HIP error code.
@@ -91,14 +109,24 @@ int launch_import(const DevDomain& d, const StepArgs& a, const double* aos,
 int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
                    double* u, void* stream);
 // Wall shear stress magnitude at nw wall sites (compact index site[i],
-// normal[3*i..]) of a binary64 SoA state; temp = -sqrt(2) * stress parameter.
+// normal[3*i..]) of a binary64 SoA state; temp[i] = -sqrt(2) * the stress
+// parameter of the site's relaxation time.
 int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
-                             const int64_t* site, const float* normal, double temp,
+                             const int64_t* site, const float* normal, const double* temp,
                              double* out, void* stream);
 // Fixed-order binary64 reductions over moments; partial results per block.
 int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
                   const double* rho, const double* u, double* partials,
                   int nblocks, void* stream);
+// Per-chunk statistics of sites [range[2c], range[2c+1]): partials[5c..]
+// = mass, min rho, max rho, max |u|, unstable; velocity change per chunk
+// with its own 1/reference speed.
+int launch_reduce_ranges(int Q, int64_t nchunks, int64_t np, const int64_t* range,
+                         const double* soa, const double* rho, const double* u,
+                         double* partials, void* stream);
+int launch_velocity_change_ranges(int64_t nchunks, int64_t np, const int64_t* range,
+                                  const double* u, const double* u_ref,
+                                  const double* inv_ref, double* partials, void* stream);
 int launch_velocity_change(int64_t n, int64_t np, const double* u,
                            const double* u_ref, double inv_ref, double* partials,
                            int nblocks, void* stream);

@@ -11,11 +11,13 @@
 // evaluated in the reference's SSE3 order without contraction, so the plain
 // method reproduces the reference bit for bit (up to the sign of zero).
 //
-// The bandwidth-bound path uses no MFMA.  Workgroups are remapped so that
-// each XCD sweeps one contiguous eighth of the Morton-ordered domain, which
-// keeps neighbour cache lines inside one L2.
+// The bandwidth-bound path uses no MFMA.  By default consecutive workgroups
+// (dispatched round-robin over the XCDs) cover consecutive x planes of the
+// Morton-ordered domain; an XCD-contiguous remap is one of the launch shapes
+// the tuner measures.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "lbm_internal.h"
@@ -388,6 +390,7 @@ struct SparseLayout {
   static __device__ __forceinline__ int64_t compact(const DevDomain&, const SiteRef& r) {
     return r.x;
   }
+  static __device__ __forceinline__ void relax(const DevDomain&, const SiteRef&, StepArgs&) {}
 };
 
 // Block layout: every 8^3 block holding fluid owns 512 slots (x-plane major,
@@ -398,7 +401,13 @@ struct SparseLayout {
 struct BlockLayout {
   static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
     const int64_t slot = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
-    r.w = __builtin_amdgcn_readfirstlane((uint32_t)(slot >> 6));
+    return locate_wave(D, __builtin_amdgcn_readfirstlane((uint32_t)(slot >> 6)), r);
+  }
+  // The lane's site in wave (x plane) w; false for lanes without a fluid
+  // site and for w beyond the domain.
+  static __device__ __forceinline__ bool locate_wave(const DevDomain& D, uint32_t w,
+                                                     SiteRef& r) {
+    r.w = w;
     r.lane = threadIdx.x & 63u;
     r.x = ((int64_t)r.w << 6) + r.lane;
     if (r.w >= D.nwaves) return false;
@@ -444,7 +453,104 @@ struct BlockLayout {
     const uint64_t fm = D.wave_fluid[r.w];
     return (int64_t)D.wave_cbase[r.w] + __popcll(fm & ((1ull << r.lane) - 1ull));
   }
+  // Ensembles relax each block with its instance's rates (one scalar load
+  // per wave); otherwise the StepArgs values hold everywhere.
+  static __device__ __forceinline__ void relax(const DevDomain& D, const SiteRef& r,
+                                               StepArgs& a) {
+    if (D.blk_omega) {
+      const uint32_t b = r.w >> 3;
+      a.omega = D.blk_omega[b];
+      a.omega_f = D.blk_omega_f[b];
+      a.mrt_s_tau = D.blk_s_tau[b];
+    }
+  }
 };
+
+// Compact-block layout: the waves, masks and neighbour-block table of the
+// block layout, but the distributions are stored densely by compact site
+// (Morton block order, z fastest), so that no cache line carries empty
+// slots.  The site of lane s of plane w is wave_cbase[w] + popcount of the
+// plane's fluid mask below s; a neighbour's index comes from the same count
+// on the source plane, whose mask and base are wave-uniform scalar loads.
+struct CompactBlockLayout {
+  static __device__ __forceinline__ int64_t site_of(uint64_t mask, uint32_t base,
+                                                    uint32_t lane) {
+    return (int64_t)base + __popcll(mask & ((1ull << lane) - 1ull));
+  }
+  static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
+    const int64_t slot = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
+    return locate_wave(D, __builtin_amdgcn_readfirstlane((uint32_t)(slot >> 6)), r);
+  }
+  static __device__ __forceinline__ bool locate_wave(const DevDomain& D, uint32_t w,
+                                                     SiteRef& r) {
+    r.w = w;
+    r.lane = threadIdx.x & 63u;
+    if (r.w >= D.nwaves) return false;
+    const uint64_t fm = D.wave_fluid[r.w];
+    const uint64_t bm = D.wave_bnd[r.w];
+    const uint32_t rb = D.wave_rec[r.w];
+    if (!((fm >> r.lane) & 1ull)) return false;
+    r.x = site_of(fm, D.wave_cbase[r.w], r.lane);
+    const int32_t* nbt = D.bnbr + (int64_t)(r.w >> 3) * 27;
+    sfor<0, 27>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      r.tb[i] = nbt[i];
+    });
+    r.cut = bm ? D.slot_cut[r.x] : 0u;
+    r.rec = rb + (uint32_t)__popcll(bm & ((1ull << r.lane) - 1ull));
+    return true;
+  }
+  // Fluid mask and compact base of x plane px of block nb (nb < 0: absent,
+  // and then no lane reads it).
+  static __device__ __forceinline__ void plane(const DevDomain& D, int32_t nb, int px,
+                                               uint64_t& m, uint32_t& b) {
+    if (nb >= 0) {
+      const uint32_t pw = ((uint32_t)nb << 3) | (uint32_t)px;
+      m = D.wave_fluid[pw];
+      b = D.wave_cbase[pw];
+    } else {
+      m = 0;
+      b = 0;
+    }
+  }
+  template <int Q, int E>
+  static __device__ __forceinline__ int64_t offset(const DevDomain& D, const SiteRef& r) {
+    constexpr int CX = Lat<Q>::c[E][0], CY = Lat<Q>::c[E][1], CZ = Lat<Q>::c[E][2];
+    const int lx = (int)(r.w & 7u);
+    const int ly = (int)(r.lane >> 3), lz = (int)(r.lane & 7u);
+    const bool bx = CX != 0 && (CX > 0 ? lx == 0 : lx == 7);  // wave-uniform
+    const bool by = CY != 0 && (CY > 0 ? ly == 0 : ly == 7);
+    const bool bz = CZ != 0 && (CZ > 0 ? lz == 0 : lz == 7);
+    constexpr int i000 = 13, i100 = 13 - 9 * CX, i010 = 13 - 3 * CY, i001 = 13 - CZ;
+    constexpr int i110 = 13 - 9 * CX - 3 * CY, i101 = 13 - 9 * CX - CZ;
+    constexpr int i011 = 13 - 3 * CY - CZ, i111 = 13 - 9 * CX - 3 * CY - CZ;
+    const int px = (lx - CX) & 7;
+    uint64_t m00, m10 = 0, m01 = 0, m11 = 0;
+    uint32_t b00, b10 = 0, b01 = 0, b11 = 0;
+    plane(D, bx ? r.tb[i100] : r.tb[i000], px, m00, b00);
+    if constexpr (CY != 0) plane(D, bx ? r.tb[i110] : r.tb[i010], px, m10, b10);
+    if constexpr (CZ != 0) plane(D, bx ? r.tb[i101] : r.tb[i001], px, m01, b01);
+    if constexpr (CY != 0 && CZ != 0) plane(D, bx ? r.tb[i111] : r.tb[i011], px, m11, b11);
+    const uint64_t m = by ? (bz ? m11 : m10) : (bz ? m01 : m00);
+    const uint32_t b = by ? (bz ? b11 : b10) : (bz ? b01 : b00);
+    const uint32_t sl = (uint32_t)((((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
+    const int64_t src = site_of(m, b, sl);
+    constexpr int DP = inv_dir(E);
+    const int64_t own = r.x + (int64_t)(DP - E) * D.np;
+    return ((r.cut >> (DP - 1)) & 1u) ? own : src;
+  }
+  static __device__ __forceinline__ int64_t compact(const DevDomain&, const SiteRef& r) {
+    return r.x;
+  }
+  static __device__ __forceinline__ void relax(const DevDomain& D, const SiteRef& r,
+                                               StepArgs& a) {
+    BlockLayout::relax(D, r, a);
+  }
+};
+
+template <class Lay>
+constexpr bool kWaveLayout = std::is_same<Lay, BlockLayout>::value ||
+                             std::is_same<Lay, CompactBlockLayout>::value;
 
 // ---------------------------------------------------------------------------
 // Streaming (pull) with boundary links
@@ -609,12 +715,10 @@ __device__ __forceinline__ void store_out(S* p, typename Store<S>::T v) {
 }
 
 template <int Q, int KERNEL, typename S, class Lay>
-__global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
-                                              const S* __restrict__ fin,
-                                              S* __restrict__ fout) {
+__device__ __forceinline__ void step_site(const DevDomain& D, const StepArgs& A,
+                                          const S* __restrict__ fin,
+                                          S* __restrict__ fout, const SiteRef& r) {
   using T = typename Store<S>::T;
-  SiteRef r;
-  if (!Lay::locate(D, r)) return;
   T f[Q];
   gather<Q, S, T, Lay>(D, A, fin, r, f);
   bool ok = true;
@@ -626,12 +730,53 @@ __global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
       ok = ok && (f[d] > T(0.0));
   });
   if (!ok) atomicOr(A.unstable, 1);
-  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, A);
+  StepArgs a = A;
+  Lay::relax(D, r, a);
+  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, a);
   update_stash<Q, T>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
     store_out<S>(fout + (int64_t)d * D.np + r.x, f[d]);
   });
+}
+
+// One timestep, one site per thread.  Block layout: one x plane per wave,
+// workgroups of A.threads, optionally remapped to XCD-contiguous ranges.
+template <int Q, int KERNEL, typename S, class Lay>
+__global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
+                                              const S* __restrict__ fin,
+                                              S* __restrict__ fout) {
+  SiteRef r;
+  if constexpr (kWaveLayout<Lay>) {
+    const uint32_t g = A.xcd_remap ? logical_workgroup() : blockIdx.x;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(g * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (!Lay::locate_wave(D, w, r)) return;
+  } else {
+    if (!Lay::locate(D, r)) return;
+  }
+  step_site<Q, KERNEL, S, Lay>(D, A, fin, fout, r);
+}
+
+// The same timestep for the block layout with a wave loop: each wave takes
+// A.planes consecutive x planes and a grid of A.groups workgroups strides
+// over the domain (a persistent launch).  The loop keeps lane-derived
+// offsets live across iterations, so it needs more registers than k_step;
+// the launch-shape tuner measures both.
+template <int Q, int KERNEL, typename S, class Lay>
+__global__ __launch_bounds__(256) void k_step_loop(DevDomain D, StepArgs A,
+                                                   const S* __restrict__ fin,
+                                                   S* __restrict__ fout) {
+  const uint32_t wpg = blockDim.x >> 6;
+  const uint32_t g = A.xcd_remap ? logical_workgroup() : blockIdx.x;
+  const uint32_t gw = __builtin_amdgcn_readfirstlane(g * wpg + (threadIdx.x >> 6));
+  const uint32_t stride = gridDim.x * wpg * A.planes;
+  for (uint32_t base = gw * A.planes; base < D.nwaves; base += stride) {
+    for (uint32_t i = 0; i < A.planes; ++i) {
+      SiteRef r;
+      if (Lay::locate_wave(D, base + i, r))
+        step_site<Q, KERNEL, S, Lay>(D, A, fin, fout, r);
+    }
+  }
 }
 
 template <int Q, int KERNEL, typename S, class Lay>
@@ -646,7 +791,9 @@ __global__ __launch_bounds__(256) void k_collide_only(DevDomain D, StepArgs A,
     constexpr int d = decltype(di)::value;
     f[d] = Store<S>::ld(fin + (int64_t)d * D.np + r.x);
   });
-  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, A);
+  StepArgs a = A;
+  Lay::relax(D, r, a);
+  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, a);
   update_stash<Q, T>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
@@ -778,6 +925,71 @@ __global__ __launch_bounds__(256) void k_reduce(int64_t n, int64_t np,
   }
 }
 
+// The same statistics per chunk of sites [range[2b], range[2b+1]) (one
+// workgroup per chunk; chunks never mix instances of an ensemble).
+template <int Q>
+__global__ __launch_bounds__(256) void k_reduce_ranges(int64_t np, const int64_t* range,
+                                                       const double* soa,
+                                                       const double* rho,
+                                                       const double* u,
+                                                       double* partials) {
+  __shared__ double s_sum[256], s_min[256], s_max[256], s_u[256], s_bad[256];
+  const int t = threadIdx.x;
+  double sum = 0.0, mn = 1.7976931348623157e308, mx = -1.7976931348623157e308,
+         um = 0.0, bad = 0.0;
+  const int64_t end = range[2 * blockIdx.x + 1];
+  for (int64_t x = range[2 * blockIdx.x] + t; x < end; x += blockDim.x) {
+    const double r = rho[x];
+    sum += r;
+    mn = r < mn ? r : mn;
+    mx = r > mx ? r : mx;
+    const double a = u[x], b = u[np + x], c = u[2 * np + x];
+    const double m = sqrt(((0.0 + a * a) + b * b) + c * c);
+    um = m > um ? m : um;
+    for (int d = 0; d < Q; ++d)
+      if (!(soa[(int64_t)d * np + x] > 0.0)) bad = 1.0;
+  }
+  s_sum[t] = sum; s_min[t] = mn; s_max[t] = mx; s_u[t] = um; s_bad[t] = bad;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s_sum[t] += s_sum[t + w];
+      s_min[t] = s_min[t + w] < s_min[t] ? s_min[t + w] : s_min[t];
+      s_max[t] = s_max[t + w] > s_max[t] ? s_max[t + w] : s_max[t];
+      s_u[t] = s_u[t + w] > s_u[t] ? s_u[t + w] : s_u[t];
+      s_bad[t] = s_bad[t + w] > s_bad[t] ? s_bad[t + w] : s_bad[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* p = partials + 5 * blockIdx.x;
+    p[0] = s_sum[0]; p[1] = s_min[0]; p[2] = s_max[0]; p[3] = s_u[0]; p[4] = s_bad[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_velocity_change_ranges(
+    int64_t np, const int64_t* range, const double* u, const double* u0,
+    const double* inv_ref, double* partials) {
+  __shared__ double s[256];
+  const int t = threadIdx.x;
+  double m = 0.0;
+  const double ir = inv_ref[blockIdx.x];
+  const int64_t end = range[2 * blockIdx.x + 1];
+  for (int64_t x = range[2 * blockIdx.x] + t; x < end; x += blockDim.x) {
+    const double a = u[x] - u0[x], b = u[np + x] - u0[np + x],
+                 c = u[2 * np + x] - u0[2 * np + x];
+    const double v = sqrt(((0.0 + a * a) + b * b) + c * c) * ir;
+    m = v > m ? v : m;
+  }
+  s[t] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) s[t] = s[t + w] > s[t] ? s[t + w] : s[t];
+    __syncthreads();
+  }
+  if (t == 0) partials[blockIdx.x] = s[0];
+}
+
 __global__ __launch_bounds__(256) void k_velocity_change(int64_t n, int64_t np,
                                                          const double* u,
                                                          const double* u0,
@@ -824,7 +1036,7 @@ template <int Q>
 __global__ __launch_bounds__(256) void k_wall_shear_stress(
     int64_t nw, int64_t np, const double* __restrict__ soa,
     const int64_t* __restrict__ site, const float* __restrict__ nrm,
-    double temp, double* __restrict__ out) {
+    const double* __restrict__ temp_of, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nw) return;
   const int64_t x = site[i];
@@ -846,6 +1058,7 @@ __global__ __launch_bounds__(256) void k_wall_shear_stress(
                p21 = second_moment<Q, 2, 1>(fneq), p22 = second_moment<Q, 2, 2>(fneq);
   const double pi[3][3] = {{p00, p10, p20}, {p10, p11, p21}, {p20, p21, p22}};
   const double n[3] = {(double)nrm[3 * i], (double)nrm[3 * i + 1], (double)nrm[3 * i + 2]};
+  const double temp = temp_of[i];
   double sq = 0.0, nn = 0.0;
   for (int a = 0; a < 3; ++a) {
     double t = 0.0;
@@ -874,7 +1087,7 @@ inline dim3 grid_for(int64_t n, int threads = 256) {
 
 // Threads launched for a per-site kernel: compact sites or all slots.
 inline int64_t thread_count(const DevDomain& d) {
-  return d.layout == 0 ? d.n : (int64_t)d.nwaves * 64;
+  return d.layout == 0 ? d.n : (int64_t)d.nwaves * 64;  // sites, or lanes of all planes
 }
 
 // (layout, lattice, kernel, storage) -> instantiation
@@ -883,7 +1096,10 @@ int dispatch(const DevDomain& d, const StepArgs& a, Fn&& fn) {
 #define HLB_CASE(L_, Q_, K_, S_, SC_)                                        \
   if (d.layout == L_ && a.Q == Q_ && a.kernel == K_ && a.storage == SC_)    \
     return fn(std::integral_constant<int, Q_>{}, std::integral_constant<int, K_>{}, \
-              (S_*)nullptr, (std::conditional_t<L_ == 0, SparseLayout, BlockLayout>*)nullptr);
+              (S_*)nullptr,                                                  \
+              (std::conditional_t<L_ == 0, SparseLayout,                     \
+                                  std::conditional_t<L_ == 1, BlockLayout,    \
+                                                     CompactBlockLayout>>*)nullptr);
 #define HLB_CASES(L_)                                                        \
   HLB_CASE(L_, 15, 0, double, 0) HLB_CASE(L_, 15, 1, double, 0)             \
   HLB_CASE(L_, 19, 0, double, 0) HLB_CASE(L_, 19, 1, double, 0)             \
@@ -893,6 +1109,7 @@ int dispatch(const DevDomain& d, const StepArgs& a, Fn&& fn) {
   HLB_CASE(L_, 19, 0, _Float16, 2) HLB_CASE(L_, 19, 1, _Float16, 2)
   HLB_CASES(0)
   HLB_CASES(1)
+  HLB_CASES(2)
 #undef HLB_CASES
 #undef HLB_CASE
   return (int)hipErrorInvalidValue;
@@ -916,9 +1133,25 @@ int launch_step(const DevDomain& d, const StepArgs& a, const void* fin, void* fo
   return dispatch(d, a, [&](auto q, auto k, auto* s, auto* lay) {
     using S = std::remove_pointer_t<decltype(s)>;
     using Lay = std::remove_pointer_t<decltype(lay)>;
-    hipLaunchKernelGGL((k_step<decltype(q)::value, decltype(k)::value, S, Lay>),
-                       grid_for(thread_count(d)), dim3(256), 0, (hipStream_t)stream, d, a,
-                       (const S*)fin, (S*)fout);
+    if constexpr (kWaveLayout<Lay>) {
+      // launch shape: a.threads per workgroup, a.planes per wave, a grid of
+      // a.groups workgroups (0: enough to cover every plane once)
+      const uint32_t wpg = a.threads / 64u;
+      const uint64_t need = ((uint64_t)d.nwaves + (uint64_t)wpg * a.planes - 1) /
+                            ((uint64_t)wpg * a.planes);
+      const uint64_t groups = a.groups ? std::min<uint64_t>(a.groups, need) : need;
+      const dim3 grid((unsigned)std::max<uint64_t>(groups, 1)), block(a.threads);
+      if (a.planes == 1 && groups == need)
+        hipLaunchKernelGGL((k_step<decltype(q)::value, decltype(k)::value, S, Lay>), grid,
+                           block, 0, (hipStream_t)stream, d, a, (const S*)fin, (S*)fout);
+      else
+        hipLaunchKernelGGL((k_step_loop<decltype(q)::value, decltype(k)::value, S, Lay>), grid,
+                           block, 0, (hipStream_t)stream, d, a, (const S*)fin, (S*)fout);
+    } else {
+      hipLaunchKernelGGL((k_step<decltype(q)::value, decltype(k)::value, S, Lay>),
+                         grid_for(thread_count(d)), dim3(256), 0, (hipStream_t)stream, d, a,
+                         (const S*)fin, (S*)fout);
+    }
     return (int)hipGetLastError();
   });
 }
@@ -977,7 +1210,7 @@ int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
 }
 
 int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
-                             const int64_t* site, const float* normal, double temp,
+                             const int64_t* site, const float* normal, const double* temp,
                              double* out, void* stream) {
   if (nw == 0) return 0;
   if (Q == 15)
@@ -998,6 +1231,28 @@ int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
   else
     hipLaunchKernelGGL(k_reduce<19>, dim3(nblocks), dim3(256), 0,
                        (hipStream_t)stream, n, np, soa, rho, u, partials);
+  return (int)hipGetLastError();
+}
+
+int launch_reduce_ranges(int Q, int64_t nchunks, int64_t np, const int64_t* range,
+                         const double* soa, const double* rho, const double* u,
+                         double* partials, void* stream) {
+  if (nchunks == 0) return 0;
+  if (Q == 15)
+    hipLaunchKernelGGL(k_reduce_ranges<15>, dim3((unsigned)nchunks), dim3(256), 0,
+                       (hipStream_t)stream, np, range, soa, rho, u, partials);
+  else
+    hipLaunchKernelGGL(k_reduce_ranges<19>, dim3((unsigned)nchunks), dim3(256), 0,
+                       (hipStream_t)stream, np, range, soa, rho, u, partials);
+  return (int)hipGetLastError();
+}
+
+int launch_velocity_change_ranges(int64_t nchunks, int64_t np, const int64_t* range,
+                                  const double* u, const double* u_ref,
+                                  const double* inv_ref, double* partials, void* stream) {
+  if (nchunks == 0) return 0;
+  hipLaunchKernelGGL(k_velocity_change_ranges, dim3((unsigned)nchunks), dim3(256), 0,
+                     (hipStream_t)stream, np, range, u, u_ref, inv_ref, partials);
   return (int)hipGetLastError();
 }
 

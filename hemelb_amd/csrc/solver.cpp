@@ -213,6 +213,10 @@ struct DevBuf {
   size_t bytes() const { return p ? sizeof(T) * (count ? count : 1) : 0; }
 };
 
+// Iolets per solver (an ensemble has two per instance); hlb_monitor
+// reports the fluxes of the first 64.
+constexpr int kMaxIolets = 4096;
+
 size_t storage_bytes(int storage) { return storage == 0 ? 8 : storage == 1 ? 4 : 2; }
 
 }  // namespace
@@ -251,6 +255,16 @@ struct hlb_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hlb::DevDomain dom;
   hlb::StepArgs args;
+  // ensemble: instances, per-block relaxation, runs of one instance in the
+  // internal order
+  struct Run { int64_t start, end; int32_t inst; };
+  int32_t n_inst = 1;
+  std::vector<double> inst_tau;
+  std::vector<Run> runs;
+  DevBuf<double> blk_omega, blk_s_tau;
+  DevBuf<float> blk_omega_f;
+  std::vector<double> wall_temp;
+  DevBuf<double> d_wall_temp;
   // wall sites (internal compact index) and their normals, for the wall
   // shear stress output
   bool has_normals = false;
@@ -379,9 +393,12 @@ double hlb_pressure_gradient_to_lattice(double pg, double dt, double dx, double 
   return pg * kMmHgToPa * dx / lattice_pressure(dt, dx, rho);
 }
 
-int hlb_create(const hlb_params* params, const hlb_domain* dm,
-               const hlb_iolet* inlets, int32_t n_inlets,
-               const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
+// Builds a solver; `instance` (optional, [n]) assigns every site to one of
+// n_inst independent vessels relaxed with inst_tau[k] (an ensemble).
+static int create_impl(const hlb_params* params, const hlb_domain* dm,
+                       const int32_t* instance, int32_t n_inst, const double* inst_tau,
+                       const hlb_iolet* inlets, int32_t n_inlets,
+                       const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
   if (!params || !dm || !out) return fail(HLB_E_INVALID, "null argument");
   *out = nullptr;
   const hlb_params& P = *params;
@@ -395,7 +412,8 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD)
       return fail(HLB_E_INVALID, "unknown iolet streamer");
   if (P.storage < 0 || P.storage > 2) return fail(HLB_E_INVALID, "unknown storage");
-  if (P.layout != HLB_LAYOUT_BLOCKS && P.layout != HLB_LAYOUT_PULL_TABLE)
+  if (P.layout != HLB_LAYOUT_BLOCKS && P.layout != HLB_LAYOUT_PULL_TABLE &&
+      P.layout != HLB_LAYOUT_COMPACT_BLOCKS)
     return fail(HLB_E_INVALID, "unknown layout");
   if (!(P.tau > 0.0) || !std::isfinite(P.tau)) return fail(HLB_E_INVALID, "tau must be positive");
   if (P.storage != HLB_STORE_FP64 && !(P.rho0 > 0.0))
@@ -406,8 +424,20 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     return fail(HLB_E_INVALID, "domain arrays missing");
   if (P.wall == HLB_WALL_BFL && !dm->cut_distance)
     return fail(HLB_E_INVALID, "BFL walls need cut distances");
-  if (n_inlets < 0 || n_outlets < 0 || n_inlets + n_outlets > 64)
-    return fail(HLB_E_INVALID, "between 0 and 64 iolets are supported");
+  if (n_inlets < 0 || n_outlets < 0 || n_inlets + n_outlets > kMaxIolets)
+    return fail(HLB_E_INVALID, "between 0 and " + std::to_string(kMaxIolets) +
+                                   " iolets are supported");
+  if (instance) {
+    if (P.layout == HLB_LAYOUT_PULL_TABLE)
+      return fail(HLB_E_INVALID, "ensembles need a block layout");
+    if (n_inst < 1 || !inst_tau) return fail(HLB_E_INVALID, "ensemble needs instances and their tau");
+    for (int32_t k = 0; k < n_inst; ++k)
+      if (!(inst_tau[k] > 0.0) || !std::isfinite(inst_tau[k]))
+        return fail(HLB_E_INVALID, "instance tau must be positive");
+    for (int64_t i = 0; i < dm->n_sites; ++i)
+      if (instance[i] < 0 || instance[i] >= n_inst)
+        return fail(HLB_E_INVALID, "site " + std::to_string(i) + ": instance out of range");
+  }
 
   auto s = std::make_unique<hlb_solver>();
   s->p = P;
@@ -415,7 +445,10 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   s->Q = Q;
   s->n = n;
   s->npc = (n + 255) / 256 * 256;
-  const bool blocks_layout = P.layout == HLB_LAYOUT_BLOCKS;
+  // Both block layouts build the wave tables; the compact one stores the
+  // distributions by compact site instead of by block slot.
+  const bool blocks_layout = P.layout == HLB_LAYOUT_BLOCKS || P.layout == HLB_LAYOUT_COMPACT_BLOCKS;
+  const bool compact_store = P.layout != HLB_LAYOUT_BLOCKS;
   LatticeHost L(Q);
   const uint32_t all = (1u << (Q - 1)) - 1u;
 
@@ -540,11 +573,11 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     blk_of[o] = (int64_t)blk_keys.size() - 1;
   }
   const int64_t nblocks = (int64_t)blk_keys.size();
-  s->np = blocks_layout ? nblocks * 512 : s->npc;
+  s->np = compact_store ? s->npc : nblocks * 512;
   const int64_t np = s->np;
   s->slot_of.resize(n);
   for (int64_t o = 0; o < n; ++o)
-    s->slot_of[o] = blocks_layout ? ((blk_of[o] << 9) | (int64_t)(keys[o].first & 511)) : o;
+    s->slot_of[o] = compact_store ? o : ((blk_of[o] << 9) | (int64_t)(keys[o].first & 511));
 
   // Pull table, boundary and iolet records.
   std::vector<int32_t> h_src;
@@ -575,6 +608,8 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
       if (j >= 0) {
         const int64_t ci = s->perm[j];
         pushes = !(((dm->wall_mask[ci] | dm->iolet_mask[ci]) >> (e - 1)) & 1u);
+        if (pushes && instance && instance[ci] != instance[i])
+          return fail(HLB_E_INVALID, "sites of different instances are linked");
       }
       if (own == pushes) {
         return fail(HLB_E_INVALID,
@@ -671,11 +706,13 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     h_wc.assign(nwaves, 0);
     uint32_t rec = 0;
     int64_t o = 0;
+    // block slot of internal site o (x plane major, z fastest)
+    auto bslot = [&](int64_t q) { return (blk_of[q] << 9) | (int64_t)(keys[q].first & 511); };
     for (int64_t w = 0; w < nwaves; ++w) {
       h_wr[w] = rec;
       h_wc[w] = (uint32_t)o;
-      while (o < n && (s->slot_of[o] >> 6) == w) {
-        const int lane = (int)(s->slot_of[o] & 63);
+      while (o < n && (bslot(o) >> 6) == w) {
+        const int lane = (int)(bslot(o) & 63);
         const int64_t i = s->perm[o];
         h_wf[w] |= 1ull << lane;
         if (dm->wall_mask[i] | dm->iolet_mask[i]) {
@@ -692,7 +729,7 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
     for (int64_t o2 = 0; o2 < n; ++o2) {
       const int64_t i = s->perm[o2];
       const uint32_t cutm = dm->wall_mask[i] | dm->iolet_mask[i];
-      const int64_t slot = s->slot_of[o2];
+      const int64_t slot = bslot(o2);
       const int64_t w = slot >> 6, lane = slot & 63;
       const int lx = (int)(w & 7), ly = (int)(lane >> 3), lz = (int)(lane & 7);
       const int64_t x = dm->coords[3 * i] - lo[0], y = dm->coords[3 * i + 1] - lo[1],
@@ -708,13 +745,60 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
         const int base = 13 + 9 * ox;
         const int32_t nb = cy ? (cz ? nbt[base - 3 * CY - CZ] : nbt[base - 3 * CY])
                               : (cz ? nbt[base - CZ] : nbt[base]);
-        const int64_t src = ((int64_t)nb << 9) +
-                            (((sx & 7) << 6) | (((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
+        const int sl = (((ly - CY) & 7) << 3) | ((lz - CZ) & 7);
+        int64_t src = ((int64_t)nb << 9) + (((sx & 7) << 6) | sl);
+        if (compact_store && nb >= 0) {
+          // CompactBlockLayout::offset: plane base + fluid lanes below
+          const int64_t pw = ((int64_t)nb << 3) | (sx & 7);
+          const uint64_t below = sl ? (h_wf[pw] & ((1ull << sl) - 1ull)) : 0ull;
+          src = (int64_t)h_wc[pw] + (int64_t)__builtin_popcountll(below);
+        }
         const int64_t j = lookup(x - CX, y - CY, z - CZ);
         if (nb < 0 || j < 0 || src != s->slot_of[j])
           return fail(HLB_E_INVALID, "internal error: block neighbour table inconsistent");
       }
     }
+  }
+
+  // Ensemble: every 8^3 block belongs to one instance; contiguous runs of
+  // one instance in the internal order for the per-instance reductions.
+  std::vector<double> h_bomega, h_bstau;
+  std::vector<float> h_bomega_f;
+  if (instance) {
+    std::vector<int32_t> binst(nblocks, -1);
+    for (int64_t o = 0; o < n; ++o) {
+      const int32_t k = instance[s->perm[o]];
+      int32_t& b = binst[blk_of[o]];
+      if (b >= 0 && b != k) return fail(HLB_E_INVALID, "instances must not share an 8^3 block");
+      b = k;
+    }
+    h_bomega.resize(nblocks);
+    h_bomega_f.resize(nblocks);
+    h_bstau.resize(nblocks);
+    for (int64_t b = 0; b < nblocks; ++b) {
+      h_bomega[b] = -1.0 / inst_tau[binst[b]];
+      h_bomega_f[b] = (float)h_bomega[b];
+      h_bstau[b] = 1.0 / inst_tau[binst[b]];
+    }
+    s->n_inst = n_inst;
+    s->inst_tau.assign(inst_tau, inst_tau + n_inst);
+    for (int64_t o = 0; o < n; ++o) {
+      const int32_t k = instance[s->perm[o]];
+      if (s->runs.empty() || s->runs.back().inst != k || s->runs.back().end != o)
+        s->runs.push_back({o, o + 1, k});
+      else
+        s->runs.back().end = o + 1;
+    }
+  } else {
+    s->n_inst = 1;
+    s->inst_tau.assign(1, P.tau);
+    s->runs.push_back({0, n, 0});
+  }
+  // wall shear stress factor -(1 - 1/(2 tau)) per wall site
+  for (int64_t o : s->wall_sites) {
+    const double tau = instance ? inst_tau[instance[s->perm[o]]] : P.tau;
+    const double stress_parameter = (1.0 - 1.0 / (2.0 * tau)) / std::sqrt(2.0);
+    s->wall_temp.push_back(stress_parameter * (-std::sqrt(2.0)));
   }
 
   // Device set-up.
@@ -757,6 +841,11 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   HIP_OR_FAIL(s->iolet_kind.upload(h_kind));
   HIP_OR_FAIL(s->weights.upload(std::vector<double>(L.w, L.w + Q)));
   HIP_OR_FAIL(s->d_perm.upload(s->perm));
+  if (instance) {
+    HIP_OR_FAIL(s->blk_omega.upload(h_bomega));
+    HIP_OR_FAIL(s->blk_omega_f.upload(h_bomega_f));
+    HIP_OR_FAIL(s->blk_s_tau.upload(h_bstau));
+  }
   HIP_OR_FAIL(s->unstable.alloc(1));
   HIP_OR_FAIL(hipMemset(s->unstable.p, 0, sizeof(int)));
   s->bv_cap = 512;
@@ -769,7 +858,7 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   }
 
   hlb::DevDomain& D = s->dom;
-  D.layout = blocks_layout ? 1 : 0;
+  D.layout = P.layout == HLB_LAYOUT_BLOCKS ? 1 : (P.layout == HLB_LAYOUT_COMPACT_BLOCKS ? 2 : 0);
   D.n = n;
   D.np = np;
   D.npc = s->npc;
@@ -796,6 +885,9 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   D.iolet_nf = s->iolet_nf.p;
   D.iolet_n = s->iolet_n.p;
   D.iolet_kind = s->iolet_kind.p;
+  D.blk_omega = s->blk_omega.p;
+  D.blk_omega_f = s->blk_omega_f.p;
+  D.blk_s_tau = s->blk_s_tau.p;
 
   hlb::StepArgs& A = s->args;
   A.Q = Q;
@@ -811,6 +903,22 @@ int hlb_create(const hlb_params* params, const hlb_domain* dm,
   s->initialised = false;
   *out = s.release();
   return HLB_OK;
+}
+
+int hlb_create(const hlb_params* params, const hlb_domain* domain,
+               const hlb_iolet* inlets, int32_t n_inlets,
+               const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
+  return create_impl(params, domain, nullptr, 0, nullptr, inlets, n_inlets, outlets,
+                     n_outlets, out);
+}
+
+int hlb_create_ensemble(const hlb_params* params, const hlb_domain* domain,
+                        const int32_t* instance, int32_t n_instances, const double* tau,
+                        const hlb_iolet* inlets, int32_t n_inlets,
+                        const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
+  if (!instance) return fail(HLB_E_INVALID, "null instance array");
+  return create_impl(params, domain, instance, n_instances, tau, inlets, n_inlets, outlets,
+                     n_outlets, out);
 }
 
 void hlb_destroy(hlb_solver* s) { delete s; }
@@ -969,14 +1077,15 @@ int hlb_get_wall_shear_stress(hlb_solver* s, double* stress) {
     if (!s->d_wall_sites.p) {
       HIP_OR_FAIL(s->d_wall_sites.upload(s->wall_sites));
       HIP_OR_FAIL(s->d_wall_normals.upload(s->wall_normals));
+      // per site: -sqrt(2) * the stress parameter of LbmParameters
+      // (LbmParameters.h:34-39) as CalculateWallShearStressMagnitude uses it
+      // (Lattice.h:669), from the site's tau
+      HIP_OR_FAIL(s->d_wall_temp.upload(s->wall_temp));
       HIP_OR_FAIL(s->d_wss.alloc((size_t)nw));
     }
-    // LbmParameters::CalcDerivedParams (LbmParameters.h:34-39) and the
-    // factor of CalculateWallShearStressMagnitude (Lattice.h:669).
-    const double stress_parameter = (1.0 - 1.0 / (2.0 * s->p.tau)) / std::sqrt(2.0);
-    const double temp = stress_parameter * (-std::sqrt(2.0));
     if (hlb::launch_wall_shear_stress(s->Q, nw, s->npc, s->s_soa.p, s->d_wall_sites.p,
-                                      s->d_wall_normals.p, temp, s->d_wss.p, s->stream))
+                                      s->d_wall_normals.p, s->d_wall_temp.p, s->d_wss.p,
+                                      s->stream))
       return fail(HLB_E_DEVICE, "wall shear stress kernel launch failed");
     HIP_OR_FAIL(hipMemcpyAsync(w.data(), s->d_wss.p, sizeof(double) * nw,
                                hipMemcpyDeviceToHost, s->stream));
@@ -1043,7 +1152,7 @@ int hlb_reduce(hlb_solver* s, hlb_monitor* out) {
     const int io = s->io_sites_iolet[k];
     const double* nrm = s->iolets[io].cfg.normal;
     const double un = ((0.0 + g[ni + k] * nrm[0]) + g[2 * ni + k] * nrm[1]) + g[3 * ni + k] * nrm[2];
-    m.iolet_flux[io] += g[k] * un;
+    if (io < 64) m.iolet_flux[io] += g[k] * un;
   }
   *out = m;
   return HLB_OK;
@@ -1079,6 +1188,196 @@ int hlb_velocity_change(hlb_solver* s, double ref, double* out) {
   double m = 0.0;
   for (double v : part) m = std::max(m, v);
   *out = m;
+  return HLB_OK;
+}
+
+// Chunks of at most kChunk sites inside the runs of one instance (the
+// per-instance reductions combine their partials in chunk order).
+static void instance_chunks(const hlb_solver* s, std::vector<int64_t>& range,
+                            std::vector<int32_t>& inst) {
+  constexpr int64_t kChunk = 1 << 16;
+  range.clear();
+  inst.clear();
+  for (const auto& r : s->runs)
+    for (int64_t a = r.start; a < r.end; a += kChunk) {
+      range.push_back(a);
+      range.push_back(std::min(r.end, a + kChunk));
+      inst.push_back(r.inst);
+    }
+}
+
+int32_t hlb_instance_count(const hlb_solver* s) { return s ? s->n_inst : 0; }
+
+int hlb_reduce_instances(hlb_solver* s, hlb_instance_monitor* out) {
+  if (!s || !out) return fail(HLB_E_INVALID, "null argument");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  int r = compute_moments(s);
+  if (r) return r;
+  std::vector<int64_t> range;
+  std::vector<int32_t> inst;
+  instance_chunks(s, range, inst);
+  const int64_t nc = (int64_t)inst.size();
+  DevBuf<int64_t> d_range;
+  DevBuf<double> d_part;
+  HIP_OR_FAIL(d_range.upload(range));
+  HIP_OR_FAIL(d_part.alloc((size_t)5 * std::max<int64_t>(nc, 1)));
+  if (hlb::launch_reduce_ranges(s->Q, nc, s->npc, d_range.p, s->s_soa.p, s->rho.p, s->u.p,
+                                d_part.p, s->stream))
+    return fail(HLB_E_DEVICE, "reduction kernel launch failed");
+  std::vector<double> part(5 * nc);
+  HIP_OR_FAIL(hipMemcpyAsync(part.data(), d_part.p, sizeof(double) * 5 * nc,
+                             hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  for (int32_t k = 0; k < s->n_inst; ++k) {
+    hlb_instance_monitor& m = out[k];
+    m = hlb_instance_monitor{};
+    m.density_min = 1.7976931348623157e308;
+    m.density_max = -1.7976931348623157e308;
+    m.stable = 1;
+  }
+  for (int64_t c = 0; c < nc; ++c) {
+    hlb_instance_monitor& m = out[inst[c]];
+    const double* p = &part[5 * c];
+    m.mass += p[0];
+    m.density_min = std::min(m.density_min, p[1]);
+    m.density_max = std::max(m.density_max, p[2]);
+    m.velocity_max = std::max(m.velocity_max, p[3]);
+    if (p[4] != 0.0) m.stable = 0;
+    m.sites += range[2 * c + 1] - range[2 * c];
+  }
+  return HLB_OK;
+}
+
+int hlb_velocity_change_instances(hlb_solver* s, const double* reference_speed, double* out) {
+  if (!s || !reference_speed || !out) return fail(HLB_E_INVALID, "null argument");
+  if (!s->u_snap.p) return fail(HLB_E_STATE, "no velocity snapshot");
+  for (int32_t k = 0; k < s->n_inst; ++k)
+    if (!(reference_speed[k] > 0)) return fail(HLB_E_INVALID, "reference speed must be positive");
+  int r = compute_moments(s);
+  if (r) return r;
+  std::vector<int64_t> range;
+  std::vector<int32_t> inst;
+  instance_chunks(s, range, inst);
+  const int64_t nc = (int64_t)inst.size();
+  std::vector<double> inv(nc);
+  for (int64_t c = 0; c < nc; ++c) inv[c] = 1.0 / reference_speed[inst[c]];
+  DevBuf<int64_t> d_range;
+  DevBuf<double> d_inv, d_part;
+  HIP_OR_FAIL(d_range.upload(range));
+  HIP_OR_FAIL(d_inv.upload(inv));
+  HIP_OR_FAIL(d_part.alloc((size_t)std::max<int64_t>(nc, 1)));
+  if (hlb::launch_velocity_change_ranges(nc, s->npc, d_range.p, s->u.p, s->u_snap.p, d_inv.p,
+                                         d_part.p, s->stream))
+    return fail(HLB_E_DEVICE, "velocity change kernel launch failed");
+  std::vector<double> part(nc);
+  HIP_OR_FAIL(hipMemcpyAsync(part.data(), d_part.p, sizeof(double) * nc,
+                             hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  std::fill(out, out + s->n_inst, 0.0);
+  for (int64_t c = 0; c < nc; ++c) out[inst[c]] = std::max(out[inst[c]], part[c]);
+  return HLB_OK;
+}
+
+static int check_shape(const hlb_solver* s, const hlb_launch_shape& sh) {
+  if (s->p.layout == HLB_LAYOUT_PULL_TABLE)
+    return fail(HLB_E_INVALID, "launch shapes apply to the block layout");
+  if (sh.threads != 64 && sh.threads != 128 && sh.threads != 256)
+    return fail(HLB_E_INVALID, "threads per workgroup must be 64, 128 or 256");
+  if (sh.planes < 1 || sh.planes > 8) return fail(HLB_E_INVALID, "planes per wave must be 1..8");
+  if (sh.groups < 0) return fail(HLB_E_INVALID, "workgroup count must not be negative");
+  return HLB_OK;
+}
+
+int hlb_set_launch_shape(hlb_solver* s, const hlb_launch_shape* shape) {
+  if (!s || !shape) return fail(HLB_E_INVALID, "null argument");
+  int r = check_shape(s, *shape);
+  if (r) return r;
+  s->args.threads = (uint32_t)shape->threads;
+  s->args.planes = (uint32_t)shape->planes;
+  s->args.groups = (uint32_t)shape->groups;
+  s->args.xcd_remap = shape->xcd_remap ? 1u : 0u;
+  return HLB_OK;
+}
+
+int hlb_get_launch_shape(const hlb_solver* s, hlb_launch_shape* shape) {
+  if (!s || !shape) return fail(HLB_E_INVALID, "null argument");
+  shape->threads = (int32_t)s->args.threads;
+  shape->planes = (int32_t)s->args.planes;
+  shape->groups = (int32_t)s->args.groups;
+  shape->xcd_remap = (int32_t)s->args.xcd_remap;
+  return HLB_OK;
+}
+
+int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* best_ms) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  if (s->p.layout == HLB_LAYOUT_PULL_TABLE)
+    return fail(HLB_E_INVALID, "the launch-shape tuner applies to the block layouts");
+  if (steps < 1) return fail(HLB_E_INVALID, "steps per trial must be positive");
+  // The trials advance the state: keep a device copy of everything a step
+  // changes (both state buffers, the iolet stash) and the time bookkeeping.
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  const size_t sbytes = storage_bytes(s->p.storage) * s->Q * s->np;
+  void* saved[2] = {nullptr, nullptr};
+  for (int b = 0; b < 2; ++b) {
+    HIP_OR_FAIL(hipMalloc(&saved[b], sbytes));
+    HIP_OR_FAIL(hipMemcpyAsync(saved[b], s->state[b], sbytes, hipMemcpyDeviceToDevice,
+                               s->stream));
+  }
+  const size_t ni = std::max<size_t>(1, s->io_sites_internal.size());
+  DevBuf<double> stash;
+  HIP_OR_FAIL(stash.alloc(ni));
+  HIP_OR_FAIL(hipMemcpyAsync(stash.p, s->io_stash.p, sizeof(double) * ni,
+                             hipMemcpyDeviceToDevice, s->stream));
+  const int cur = s->cur;
+  const uint64_t t_ref = s->t_ref;
+  const bool post_stream = s->post_stream;
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->p.device);
+  std::vector<hlb_launch_shape> cand;
+  for (int th : {256, 128, 64})
+    for (int xcd : {1, 0}) cand.push_back({th, 1, 0, xcd});
+  cand.push_back({256, 2, 0, 1});
+  cand.push_back({128, 2, 0, 1});
+  for (int th : {256, 128}) cand.push_back({th, 1, cus * 24 * 64 / th, 1});
+  cand.push_back({256, 2, cus * 24 * 64 / 256, 1});
+  hlb_launch_shape keep{};
+  hlb_get_launch_shape(s, &keep);
+  double best_t = 1e300;
+  hlb_launch_shape best_s = keep;
+  int rc = HLB_OK;
+  for (const auto& c : cand) {
+    hlb_set_launch_shape(s, &c);
+    double ms = 0;
+    rc = hlb_step(s, 2);  // warm
+    if (!rc) rc = hlb_time_steps(s, steps, &ms);
+    if (rc) break;
+    if (ms < best_t) {
+      best_t = ms;
+      best_s = c;
+    }
+  }
+  // restore
+  for (int b = 0; b < 2; ++b) {
+    if (hipMemcpyAsync(s->state[b], saved[b], sbytes, hipMemcpyDeviceToDevice, s->stream) !=
+        hipSuccess)
+      rc = fail(HLB_E_DEVICE, "state restore failed");
+  }
+  if (hipMemcpyAsync(s->io_stash.p, stash.p, sizeof(double) * ni, hipMemcpyDeviceToDevice,
+                     s->stream) != hipSuccess)
+    rc = fail(HLB_E_DEVICE, "state restore failed");
+  (void)hipStreamSynchronize(s->stream);
+  for (int b = 0; b < 2; ++b) (void)hipFree(saved[b]);
+  s->cur = cur;
+  s->t_ref = t_ref;
+  s->post_stream = post_stream;
+  if (rc) {
+    hlb_set_launch_shape(s, &keep);
+    return rc;
+  }
+  hlb_set_launch_shape(s, &best_s);
+  if (best) *best = best_s;
+  if (best_ms) *best_ms = best_t / steps;
   return HLB_OK;
 }
 

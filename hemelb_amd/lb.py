@@ -245,7 +245,8 @@ def _iolet_array(lst):
 
 
 STORAGE = {"fp64": N.HLB_STORE_FP64, "fp32": N.HLB_STORE_FP32, "fp16": N.HLB_STORE_FP16}
-LAYOUT = {"blocks": N.HLB_LAYOUT_BLOCKS, "pull_table": N.HLB_LAYOUT_PULL_TABLE}
+LAYOUT = {"blocks": N.HLB_LAYOUT_BLOCKS, "pull_table": N.HLB_LAYOUT_PULL_TABLE,
+          "compact_blocks": N.HLB_LAYOUT_COMPACT_BLOCKS}
 
 
 class LBM:
@@ -253,7 +254,10 @@ class LBM:
 
     def __init__(self, traits: Traits, domain: Domain, params: LbmParameters,
                  inlets=(), outlets=(), storage: str = "fp64", rho0: float = 1.0,
-                 device: int = 0, layout: str = "blocks"):
+                 device: int = 0, layout: str = "blocks", instances=None,
+                 instance_tau=None):
+        """`instances` ([n] int, optional) makes an ensemble: independent
+        vessels in one solver, instance k relaxed with instance_tau[k]."""
         if domain.Q != traits.Lattice.NUMVECTORS:
             raise ValueError("domain was projected for a different lattice")
         self.traits, self.domain, self.params = traits, domain, params
@@ -272,10 +276,19 @@ class LBM:
         self._inlets, self._outlets = list(inlets), list(outlets)
         self._cdom = domain.as_c()
         h = C.c_void_p()
-        N.check(N.lib().hlb_create(C.byref(p), C.byref(self._cdom),
-                                   _iolet_array(self._inlets), len(self._inlets),
-                                   _iolet_array(self._outlets), len(self._outlets),
-                                   C.byref(h)))
+        if instances is None:
+            N.check(N.lib().hlb_create(C.byref(p), C.byref(self._cdom),
+                                       _iolet_array(self._inlets), len(self._inlets),
+                                       _iolet_array(self._outlets), len(self._outlets),
+                                       C.byref(h)))
+        else:
+            self._inst = np.ascontiguousarray(instances, np.int32)
+            self._inst_tau = np.ascontiguousarray(instance_tau, np.float64)
+            N.check(N.lib().hlb_create_ensemble(
+                C.byref(p), C.byref(self._cdom), self._inst.ctypes.data_as(C.c_void_p),
+                len(self._inst_tau), self._inst_tau.ctypes.data_as(C.c_void_p),
+                _iolet_array(self._inlets), len(self._inlets),
+                _iolet_array(self._outlets), len(self._outlets), C.byref(h)))
         self._h = h
 
     def __del__(self):
@@ -337,6 +350,45 @@ class LBM:
         return {"mass": m.mass, "density_min": m.density_min,
                 "density_max": m.density_max, "velocity_max": m.velocity_max,
                 "stable": bool(m.stable), "iolet_flux": list(m.iolet_flux[:m.n_iolets])}
+
+    @property
+    def instance_count(self) -> int:
+        return int(N.lib().hlb_instance_count(self._h))
+
+    def monitor_instances(self) -> list:
+        k = self.instance_count
+        arr = (N.hlb_instance_monitor * k)()
+        N.check(N.lib().hlb_reduce_instances(self._h, arr))
+        return [{"mass": m.mass, "density_min": m.density_min, "density_max": m.density_max,
+                 "velocity_max": m.velocity_max, "stable": bool(m.stable), "sites": m.sites}
+                for m in arr]
+
+    def velocity_change_instances(self, reference_speed) -> np.ndarray:
+        ref = np.ascontiguousarray(np.broadcast_to(reference_speed, (self.instance_count,)),
+                                   np.float64)
+        out = np.empty(self.instance_count)
+        N.check(N.lib().hlb_velocity_change_instances(
+            self._h, ref.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_launch_shape(self, threads=256, planes=1, groups=0, xcd_remap=True) -> None:
+        sh = N.hlb_launch_shape(threads, planes, groups, int(bool(xcd_remap)))
+        N.check(N.lib().hlb_set_launch_shape(self._h, C.byref(sh)))
+
+    @property
+    def launch_shape(self) -> dict:
+        sh = N.hlb_launch_shape()
+        N.check(N.lib().hlb_get_launch_shape(self._h, C.byref(sh)))
+        return {"threads": sh.threads, "planes": sh.planes, "groups": sh.groups,
+                "xcd_remap": bool(sh.xcd_remap)}
+
+    def autotune(self, steps: int = 20) -> dict:
+        """Times the candidate launch shapes on the current state (which is
+        restored afterwards) and keeps the fastest."""
+        sh, ms = N.hlb_launch_shape(), C.c_double()
+        N.check(N.lib().hlb_autotune(self._h, int(steps), C.byref(sh), C.byref(ms)))
+        return {"threads": sh.threads, "planes": sh.planes, "groups": sh.groups,
+                "xcd_remap": bool(sh.xcd_remap), "ms_per_step": ms.value}
 
     def snapshot_velocity(self) -> None:
         N.check(N.lib().hlb_snapshot_velocity(self._h))

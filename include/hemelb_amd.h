@@ -70,9 +70,12 @@ enum hlb_storage { HLB_STORE_FP64 = 0, HLB_STORE_FP32 = 1, HLB_STORE_FP16 = 2 };
 /* Layout of the distributions in HBM.  HLB_LAYOUT_BLOCKS gives every 8^3
  * block that holds fluid 512 slots (Morton block order, z fastest) and finds
  * neighbours by slot arithmetic; HLB_LAYOUT_PULL_TABLE packs the fluid sites
- * and reads a 32-bit pull index per site and direction (smaller for very
- * sparse geometries).  Results are identical. */
-enum hlb_layout { HLB_LAYOUT_BLOCKS = 0, HLB_LAYOUT_PULL_TABLE = 1 };
+ * and reads a 32-bit pull index per site and direction;
+ * HLB_LAYOUT_COMPACT_BLOCKS packs the fluid sites in the same order and finds
+ * neighbours from the per-plane fluid masks of the 8^3 blocks (no index
+ * table, no empty slots).  Results are identical. */
+enum hlb_layout { HLB_LAYOUT_BLOCKS = 0, HLB_LAYOUT_PULL_TABLE = 1,
+                  HLB_LAYOUT_COMPACT_BLOCKS = 2 };
 
 typedef struct hlb_solver hlb_solver;
 
@@ -125,7 +128,7 @@ typedef struct {
  * mass = sum of density; min/max density; max |u|; `stable` is 0 if any
  * distribution is not > 0 (StabilityTester.h:91-118 criterion);
  * iolet_flux[k] = sum over the sites touching iolet k of (momentum . normal),
- * inlets first then outlets. */
+ * inlets first then outlets (the first 64 iolets; n_iolets counts all). */
 typedef struct {
   double mass;
   double density_min, density_max;
@@ -180,6 +183,53 @@ int hlb_get_macroscopic(hlb_solver* s, double* density, double* velocity);
 int hlb_get_wall_shear_stress(hlb_solver* s, double* stress);
 
 int hlb_reduce(hlb_solver* s, hlb_monitor* out);
+
+/* ---- Batched ensemble --------------------------------------------------
+ * No counterpart in the reference, which runs one vessel per job.  Several
+ * independent vessels share one solver: instance[i] in [0, n_instances)
+ * gives the vessel of site i and tau[k] its relaxation time.  The sites of
+ * different instances must not be linked and must not share an 8^3 block
+ * (keep one empty block between vessels); iolet ids index the common inlet
+ * and outlet arrays.  Needs a block layout.  One launch per step updates
+ * every instance. */
+int hlb_create_ensemble(const hlb_params* params, const hlb_domain* domain,
+                        const int32_t* instance, int32_t n_instances, const double* tau,
+                        const hlb_iolet* inlets, int32_t n_inlets,
+                        const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out);
+int32_t hlb_instance_count(const hlb_solver* s);
+
+typedef struct {
+  double mass;
+  double density_min, density_max;
+  double velocity_max;
+  int32_t stable;
+  int32_t reserved;
+  int64_t sites;
+} hlb_instance_monitor;
+
+/* hlb_reduce's statistics for every instance (out[n_instances]); a plain
+ * solver is one instance. */
+int hlb_reduce_instances(hlb_solver* s, hlb_instance_monitor* out);
+/* hlb_velocity_change per instance, each against its own reference speed. */
+int hlb_velocity_change_instances(hlb_solver* s, const double* reference_speed,
+                                  double* out);
+
+/* Launch shape of the step kernel (block layouts): threads per workgroup
+ * (64, 128, 256), x planes (64-site tiles) per wave, workgroups in the grid
+ * (0: one pass over the domain; a positive count gives a persistent grid
+ * that strides over it) and the XCD-contiguous workgroup remap (0/1).
+ * hlb_autotune times a set of shapes for `steps` steps each on the current
+ * state, restores the state and time, and keeps the fastest. */
+typedef struct {
+  int32_t threads;
+  int32_t planes;
+  int32_t groups;
+  int32_t xcd_remap;
+} hlb_launch_shape;
+int hlb_set_launch_shape(hlb_solver* s, const hlb_launch_shape* shape);
+int hlb_get_launch_shape(const hlb_solver* s, hlb_launch_shape* shape);
+int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best,
+                 double* best_ms_per_step);
 
 /* Convergence monitor: hlb_snapshot_velocity stores the current velocity
  * field on the device; hlb_velocity_change returns

@@ -105,3 +105,40 @@ def test_parameter_helpers_match_oracle():
             O.lib().orc_pressure_diff_to_lattice(p, dt, dx, rho)
         assert lib.hlb_pressure_gradient_to_lattice(p, dt, dx, rho) == \
             O.lib().orc_pressure_gradient_to_lattice(p, dt, dx, rho)
+
+
+def _create_ensemble(e, instance):
+    p = N.hlb_params()
+    p.lattice, p.kernel, p.wall = 19, N.HLB_LBGK, N.HLB_WALL_BFL
+    p.inlet_bc = p.outlet_bc = N.HLB_IOLET_NASH
+    p.storage, p.tau, p.rho0, p.device, p.layout = 1, 0.8, 1.0, 0, 0
+    cd = e.domain.as_c()
+    inst = np.ascontiguousarray(instance, np.int32)
+    taus = np.ascontiguousarray(e.taus, np.float64)
+    ins, outs = L._iolet_array(e.inlets), L._iolet_array(e.outlets)
+    h = C.c_void_p()
+    rc = N.lib().hlb_create_ensemble(C.byref(p), C.byref(cd), inst.ctypes.data_as(C.c_void_p),
+                                     len(taus), taus.ctypes.data_as(C.c_void_p), ins,
+                                     len(e.inlets), outs, len(e.outlets), C.byref(h))
+    err = N.lib().hlb_last_error().decode()
+    if rc == N.HLB_OK:
+        N.lib().hlb_destroy(h)
+    return rc, err
+
+
+def test_ensemble_validation():
+    """Instances must be separate vessels: linked sites or a shared 8^3
+    block are rejected before any device work."""
+    from hemelb_amd import workloads as WL
+    e = WL.stenosed_ensemble(3, seed=1, radius_range=(4.0, 6.0))
+    assert len(e.inlets) == len(e.outlets) == 3
+    rc, err = _create_ensemble(e, e.instance_of_site)
+    assert rc in (N.HLB_OK, N.HLB_E_DEVICE), err     # valid (no GPU here)
+    bad = e.instance_of_site.copy()
+    bad[e.instances[0].sites.start:e.instances[0].sites.start + 40] = 1
+    rc, err = _create_ensemble(e, bad)
+    assert rc == N.HLB_E_INVALID and "instances" in err
+    bad = e.instance_of_site.copy()
+    bad[0] = 7
+    rc, err = _create_ensemble(e, bad)
+    assert rc == N.HLB_E_INVALID and "instance out of range" in err

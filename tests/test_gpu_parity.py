@@ -50,7 +50,10 @@ PARITY_CASES = [
 ]
 
 
-@pytest.mark.parametrize("layout", ["blocks", "pull_table"])
+LAYOUTS = ["blocks", "pull_table", "compact_blocks"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("label,make,steps", PARITY_CASES, ids=[c[0] for c in PARITY_CASES])
 def test_parity_bitwise(label, make, steps, layout):
     _run_and_compare(make(), steps, chunks=[1, 2, steps - 3], label=label, layout=layout)
@@ -147,7 +150,7 @@ def test_convergence_monitor():
     np.testing.assert_allclose(gpu.velocity_change(0.01), expect, rtol=1e-14)
 
 
-@pytest.mark.parametrize("layout", ["blocks", "pull_table"])
+@pytest.mark.parametrize("layout", LAYOUTS)
 def test_halo_pack_returns_pending_post_collision_values(layout):
     """After n steps the state holds the post-collision values of step n whose
     streaming is still pending: they equal the collision of the oracle's state

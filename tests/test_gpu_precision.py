@@ -56,8 +56,9 @@ def test_reduced_storage_error_against_binary64(label, make, steps, storage):
 def test_reduced_storage_layouts_agree_bitwise(storage):
     case = K.tree_case(3, 6.0, seed=1)
     a, _ = _fields(case, storage, 30, "blocks")
-    b, _ = _fields(case, storage, 30, "pull_table")
-    np.testing.assert_array_equal(a.get_distributions(), b.get_distributions())
+    for layout in ("pull_table", "compact_blocks"):
+        b, _ = _fields(case, storage, 30, layout)
+        np.testing.assert_array_equal(a.get_distributions(), b.get_distributions(), layout)
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
