@@ -111,7 +111,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--storage", choices=sorted(STORE_BYTES), default="fp64")
-    ap.add_argument("--layout", choices=["blocks", "pull_table"], default="blocks")
+    ap.add_argument("--layout", choices=["blocks", "pull_table", "compact_blocks"],
+                    default="blocks")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",

@@ -138,6 +138,7 @@ _SIGS = {
     "hlbg_get_iolet": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(hlbg_iolet)]),
     "hlbg_get_links": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "hlbg_project": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "hlbg_signed_distance": (C.c_int, [_P, _P, _P, _P]),
     "hlbg_last_error": (C.c_char_p, []),
     # extraction files
     "hlbx_create": (C.c_int, [C.c_char_p, C.c_double, _P, C.c_int64, _P, C.c_int32,

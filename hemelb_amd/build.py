@@ -1,9 +1,10 @@
 """Builds ``hemelb_amd/libhemelb_amd.so`` in-tree.
 
-* ``csrc/lbm_kernels.hip`` -> hipcc for gfx950 (device code; no FMA
-  contraction so binary64 results match the reference's evaluation order).
-* ``csrc/solver.cpp``, ``csrc/geometry.cpp`` and ``csrc/extraction.cpp`` ->
-  g++ (host code; the same compiler runtime and libm as the reference's own
+* ``csrc/lbm_kernels.hip`` and ``csrc/unet_kernels.hip`` -> hipcc for gfx950
+  (device code; no FMA contraction so binary64 results match the
+  reference's evaluation order).
+* ``csrc/solver.cpp``, ``csrc/geometry.cpp``, ``csrc/extraction.cpp`` and
+  ``csrc/unet.cpp`` -> g++ (host code; the same compiler runtime and libm as the reference's own
   g++ build).
 * linked with g++ against libamdhip64 and zlib.
 
@@ -51,11 +52,12 @@ def _run(cmd):
 def build() -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    k_obj = os.path.join(BUILD, "lbm_kernels.o")
-    _run([os.path.join(ROCM, "bin", "hipcc")] + HIPCC_FLAGS +
-         ["-c", os.path.join(CSRC, "lbm_kernels.hip"), "-o", k_obj])
-    objs.append(k_obj)
-    for src in ("solver.cpp", "geometry.cpp", "extraction.cpp"):
+    for hip in ("lbm_kernels.hip", "unet_kernels.hip"):
+        k_obj = os.path.join(BUILD, hip.replace(".hip", ".o"))
+        _run([os.path.join(ROCM, "bin", "hipcc")] + HIPCC_FLAGS +
+             ["-c", os.path.join(CSRC, hip), "-o", k_obj])
+        objs.append(k_obj)
+    for src in ("solver.cpp", "geometry.cpp", "extraction.cpp", "unet.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++"] + GXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -545,6 +545,81 @@ extern "C" {
 
 const char* hlbg_last_error(void) { return g_err.c_str(); }
 
+namespace {
+// Squared Euclidean distance transform of a sampled function along one line
+// (lower envelope of parabolas; Felzenszwalb & Huttenlocher, "Distance
+// Transforms of Sampled Functions", Theory of Computing 8, 2012).
+void edt_line(const double* f, int n, double* d, int* v, double* z) {
+  int k = 0;
+  v[0] = 0;
+  z[0] = -1e300;
+  z[1] = 1e300;
+  for (int q = 1; q < n; ++q) {
+    double s = ((f[q] + (double)q * q) - (f[v[k]] + (double)v[k] * v[k])) / (2.0 * (q - v[k]));
+    while (s <= z[k]) {  // z[0] = -inf ends the loop at k = 0
+      --k;
+      s = ((f[q] + (double)q * q) - (f[v[k]] + (double)v[k] * v[k])) / (2.0 * (q - v[k]));
+    }
+    ++k;
+    v[k] = q;
+    z[k] = s;
+    z[k + 1] = 1e300;
+  }
+  k = 0;
+  for (int q = 0; q < n; ++q) {
+    while (z[k + 1] < q) ++k;
+    const double dq = q - v[k];
+    d[q] = dq * dq + f[v[k]];
+  }
+}
+
+// Squared distance from every voxel to the nearest voxel with seed != 0.
+void edt3(const std::vector<uint8_t>& seed, const int32_t* dims, std::vector<double>& out) {
+  const int64_t nx = dims[0], ny = dims[1], nz = dims[2];
+  const double inf = 1e20;
+  out.assign((size_t)(nx * ny * nz), 0.0);
+  for (size_t i = 0; i < out.size(); ++i) out[i] = seed[i] ? 0.0 : inf;
+  const int m = (int)std::max(nx, std::max(ny, nz));
+  std::vector<double> f(m), d(m), z(m + 1);
+  std::vector<int> v(m);
+  auto pass = [&](int64_t n, int64_t stride, int64_t count, auto base_of) {
+    for (int64_t c = 0; c < count; ++c) {
+      const int64_t b = base_of(c);
+      for (int64_t i = 0; i < n; ++i) f[i] = out[b + i * stride];
+      edt_line(f.data(), (int)n, d.data(), v.data(), z.data());
+      for (int64_t i = 0; i < n; ++i) out[b + i * stride] = d[i];
+    }
+  };
+  // index = (x * ny + y) * nz + z
+  pass(nz, 1, nx * ny, [&](int64_t c) { return c * nz; });
+  pass(ny, nz, nx * nz, [&](int64_t c) { return (c / nz) * ny * nz + (c % nz); });
+  pass(nx, ny * nz, ny * nz, [&](int64_t c) { return c; });
+}
+}  // namespace
+
+int hlbg_signed_distance(const hlbg_geometry* g, const int32_t origin[3],
+                         const int32_t dims[3], float* out) {
+  if (!g || !origin || !dims || !out) return fail("null argument");
+  for (int a = 0; a < 3; ++a)
+    if (dims[a] < 1 || dims[a] > 4096) return fail("crop dimensions must be 1..4096");
+  const int64_t nx = dims[0], ny = dims[1], nz = dims[2];
+  std::vector<uint8_t> fluid((size_t)(nx * ny * nz), 0), solid;
+  for (int64_t s = 0; s < g->size(); ++s) {
+    const int64_t x = g->coords[3 * s] - origin[0], y = g->coords[3 * s + 1] - origin[1],
+                  z = g->coords[3 * s + 2] - origin[2];
+    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;
+    fluid[(size_t)((x * ny + y) * nz + z)] = 1;
+  }
+  solid.resize(fluid.size());
+  for (size_t i = 0; i < fluid.size(); ++i) solid[i] = !fluid[i];
+  std::vector<double> to_solid, to_fluid;
+  edt3(solid, dims, to_solid);
+  edt3(fluid, dims, to_fluid);
+  for (size_t i = 0; i < fluid.size(); ++i)
+    out[i] = fluid[i] ? (float)std::sqrt(to_solid[i]) : -(float)std::sqrt(to_fluid[i]);
+  return 0;
+}
+
 void hlbg_free(hlbg_geometry* g) { delete g; }
 
 int64_t hlbg_fluid_site_count(const hlbg_geometry* g) { return g ? g->size() : -1; }

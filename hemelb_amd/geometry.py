@@ -74,6 +74,16 @@ class Geometry:
             N.lib().hlbg_free(self._h)
             self._h = C.c_void_p()
 
+    def signed_distance(self, origin, dims) -> np.ndarray:
+        """Signed distance field of the crop [origin, origin + dims) in voxels:
+        positive at fluid sites (distance to the nearest non-fluid voxel),
+        negative elsewhere (distance to the nearest fluid site); [x][y][z]."""
+        o = np.ascontiguousarray(origin, np.int32)
+        d = np.ascontiguousarray(dims, np.int32)
+        out = np.empty(tuple(int(v) for v in d), np.float32)
+        N.check(N.lib().hlbg_signed_distance(self._h, _ptr(o), _ptr(d), _ptr(out)), True)
+        return out
+
     # ---- constructors ----
     @classmethod
     def cylinder(cls, radius: float, length: int, block_size: int = 8) -> "Geometry":

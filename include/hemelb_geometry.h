@@ -99,6 +99,15 @@ int hlbg_project(const hlbg_geometry* g, int32_t Q, int32_t* coords,
                  uint32_t* site_type, uint32_t* wall_mask, uint32_t* iolet_mask,
                  int32_t* iolet_id, float* cut, float* normal);
 
+/* Signed distance field of the crop [origin, origin + dims) (voxel centres,
+ * out[(x * dims[1] + y) * dims[2] + z]): at a fluid site the Euclidean
+ * distance to the nearest non-fluid voxel, elsewhere minus the distance to
+ * the nearest fluid site (voxel units; a fully solid or fully fluid crop
+ * gives +-1e10).  The input of the learned warm start; no counterpart in
+ * the reference. */
+int hlbg_signed_distance(const hlbg_geometry* g, const int32_t origin[3],
+                         const int32_t dims[3], float* out);
+
 const char* hlbg_last_error(void);
 
 #ifdef __cplusplus
