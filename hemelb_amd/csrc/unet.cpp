@@ -220,6 +220,11 @@ int hlbn_create(int32_t side, int32_t batch, int32_t in_channels, int32_t out_ch
     return fail("side must be a positive multiple of 8; 1..16 input and output channels");
   for (int k = 0; k < 4; ++k)
     if (chans[k] < 16 || chans[k] % 16) return fail("channel counts must be multiples of 16");
+  // the backward-weight kernel indexes activations with 32-bit offsets
+  const int64_t widest = std::max<int64_t>({chans[1] + chans[0], (chans[2] + chans[1]) / 8,
+                                            (chans[3] + chans[2]) / 64, chans[3] / 512});
+  if ((int64_t)batch * side * side * side * widest >= ((int64_t)1 << 31))
+    return fail("batch * side^3 * channels must stay below 2^31");
   auto n = std::make_unique<hlbn_net>();
   n->S = side;
   n->N = batch;

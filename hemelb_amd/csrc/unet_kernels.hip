@@ -116,85 +116,74 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t* __res
 
 // --------------------------------------------------------------------------
 // Backward-weight: dW[t][co][ci] = sum_p dy[p][co] * x[p + off_t][ci].  Per
-// wave: one tap, MT x NT tiles (co x ci), one chunk of positions; MFMA with
-// A = dy^T (co x positions), B = shifted x (positions x ci).  Partial sums of
-// chunk k go to part[k][27][co][ci]; a second kernel adds the chunks in order.
+// wave: one chunk of positions, TG taps (one kd plane: kh, kw in 0..2), one
+// 16 x 16 (co x ci) tile; MFMA with A = dy^T (co x 16 positions), loaded once
+// per K step for all TG taps, and B = the shifted input (16 positions x ci).
+// Partial sums of chunk k go to part[k][27][co][ci]; a second kernel adds the
+// chunks in order.
 // --------------------------------------------------------------------------
-template <int MT, int NT>
+template <int TG>
 __global__ __launch_bounds__(256) void k_conv_wgrad(Grid3 g, const uint16_t* __restrict__ x,
                                                     int ci, const uint16_t* __restrict__ dy,
-                                                    int co, int64_t chunk,
+                                                    int co, int64_t chunk64,
                                                     float* __restrict__ part) {
+  // 32-bit positions and element offsets (the host keeps P * channels < 2^31)
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int mgroups = (co / 16 + MT - 1) / MT, ngroups = (ci / 16 + NT - 1) / NT;
-  const int64_t per_chunk = (int64_t)27 * mgroups * ngroups;
-  const int64_t k = wave / per_chunk;
-  const int64_t P = g.P();
-  const int64_t nchunks = (P + chunk - 1) / chunk;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int mt = co / 16, nt = ci / 16, ngr = 27 / TG;
+  const int per_chunk = ngr * mt * nt;
+  const int k = wave / per_chunk;
+  const int P = (int)g.P();
+  const int chunk = (int)chunk64;
+  const int nchunks = (P + chunk - 1) / chunk;
   if (k >= nchunks) return;
-  int64_t rem = wave % per_chunk;
-  const int t = (int)(rem / (mgroups * ngroups));
-  rem %= (mgroups * ngroups);
-  const int m0 = (int)(rem / ngroups) * MT, n0 = (int)(rem % ngroups) * NT;
-  const int mt = min(MT, co / 16 - m0), nt = min(NT, ci / 16 - n0);
+  int rem = wave % per_chunk;
+  const int tg = rem / (mt * nt);
+  rem %= (mt * nt);
+  const int m0 = rem / nt, n0 = rem % nt;
   const int row = lane & 15, kq = lane >> 4;
-  float4_t acc[MT][NT];
+  const int oc = m0 * 16 + row, ic = n0 * 16 + row;
+  float4_t acc[TG];
 #pragma unroll
-  for (int i = 0; i < MT; ++i)
-#pragma unroll
-    for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const int64_t pbeg = k * chunk, pend = min(P, pbeg + chunk);
-  for (int64_t pk = pbeg; pk < pend; pk += 16) {
-    // this lane's 4 positions (K elements)
-    int64_t pos[4], src[4];
+  for (int i = 0; i < TG; ++i) acc[i] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const int pbeg = k * chunk, pend = min(P, pbeg + chunk);
+  const int HW = g.H * g.W, DHW = g.D * HW;
+  const int kd = tg * TG / 9 - 1;  // TG = 9: one kd plane per tap group
+  for (int pk = pbeg; pk < pend; pk += 16) {
+    short4_t a = short4_t{0, 0, 0, 0};
+    int pos[4];
+    unsigned okd = 0;  // bit j: position j valid and its kd-plane neighbour inside
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       pos[j] = pk + 4 * kq + j;
-      src[j] = pos[j] < pend ? tap_neighbour(g, pos[j], t) : -1;
-    }
-    short4_t a[MT], b[NT];
-#pragma unroll
-    for (int i = 0; i < MT; ++i) {
-      const int oc = (m0 + i) * 16 + row;
-      short4_t v = short4_t{0, 0, 0, 0};
-      if (i < mt) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          v[j] = pos[j] < pend ? (short)dy[pos[j] * co + oc] : (short)0;
+      if (pos[j] < pend) {
+        a[j] = (short)dy[pos[j] * co + oc];
+        const int d2 = (pos[j] % DHW) / HW + kd;
+        if (d2 >= 0 && d2 < g.D) okd |= 1u << j;
       }
-      a[i] = v;
     }
 #pragma unroll
-    for (int jn = 0; jn < NT; ++jn) {
-      const int ic = (n0 + jn) * 16 + row;
-      short4_t v = short4_t{0, 0, 0, 0};
-      if (jn < nt) {
+    for (int tt = 0; tt < TG; ++tt) {
+      const int dyy = tt / 3 - 1, dx = tt % 3 - 1;
+      const int off = (kd * g.H + dyy) * g.W + dx;
+      short4_t b = short4_t{0, 0, 0, 0};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = src[j] >= 0 ? (short)x[src[j] * ci + ic] : (short)0;
+      for (int j = 0; j < 4; ++j) {
+        if ((okd >> j) & 1u) {
+          const int h2 = (pos[j] / g.W) % g.H + dyy, w2 = pos[j] % g.W + dx;
+          if (h2 >= 0 && h2 < g.H && w2 >= 0 && w2 < g.W) b[j] = (short)x[(pos[j] + off) * ci + ic];
+        }
       }
-      b[jn] = v;
+      acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tt], 0, 0, 0);
     }
-#pragma unroll
-    for (int i = 0; i < MT; ++i)
-#pragma unroll
-      for (int jn = 0; jn < NT; ++jn)
-        if (i < mt && jn < nt)
-          acc[i][jn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[jn], acc[i][jn], 0, 0, 0);
   }
-  float* out = part + ((k * 27 + t) * (int64_t)co) * ci;
 #pragma unroll
-  for (int i = 0; i < MT; ++i)
+  for (int tt = 0; tt < TG; ++tt) {
+    const int t = tg * TG + tt;
+    float* out = part + ((int64_t)(k * 27 + t) * co) * ci;
 #pragma unroll
-    for (int jn = 0; jn < NT; ++jn) {
-      if (i >= mt || jn >= nt) continue;
-      const int ic = (n0 + jn) * 16 + row;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int oc = (m0 + i) * 16 + 4 * kq + r;
-        out[(int64_t)oc * ci + ic] = acc[i][jn][r];
-      }
-    }
+    for (int r = 0; r < 4; ++r) out[(m0 * 16 + 4 * kq + r) * ci + ic] = acc[tt][r];
+  }
 }
 
 // out[i] = sum over k of part[k][i], k in order (deterministic).
@@ -438,7 +427,7 @@ int conv_fwd(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* w,
 
 int64_t wgrad_chunk(const ConvGeom& cg) {
   const int64_t P = (int64_t)cg.N * cg.D * cg.H * cg.W;
-  int64_t chunk = (P + 63) / 64;
+  int64_t chunk = (P + 511) / 512;
   chunk = (chunk + 15) / 16 * 16;
   return chunk < 256 ? 256 : chunk;
 }
@@ -449,12 +438,11 @@ int conv_wgrad(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* dy
   const int64_t P = g.P();
   const int64_t chunk = wgrad_chunk(cg);
   const int64_t nchunks = (P + chunk - 1) / chunk;
-  constexpr int MT = 4, NT = 4;
-  const int mg = (co / 16 + MT - 1) / MT, ng = (ci / 16 + NT - 1) / NT;
-  const int64_t waves = nchunks * 27 * mg * ng;
+  constexpr int TG = 9;
+  const int64_t waves = nchunks * (27 / TG) * (co / 16) * (ci / 16);
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL((k_conv_wgrad<MT, NT>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0,
-                     s, g, x, ci, dy, co, chunk, part);
+  hipLaunchKernelGGL((k_conv_wgrad<TG>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g,
+                     x, ci, dy, co, chunk, part);
   const int64_t n = (int64_t)27 * co * ci;
   hipLaunchKernelGGL(k_sum_chunks, blocks_for(n), dim3(256), 0, s, part, nchunks, n, dw);
   return (int)hipGetLastError();

@@ -120,6 +120,8 @@ class Instance:
     amplitude_mmHg: float      # inlet pressure amplitude
     offset: tuple              # placement of the instance in the ensemble
     sites: slice               # its rows of the ensemble domain
+    inlets: slice = slice(0, 0)   # its entries of the ensemble's inlet list
+    outlets: slice = slice(0, 0)  # and outlet list
 
 
 @dataclass
@@ -150,55 +152,44 @@ class Ensemble:
 
     def instance_domain(self, k: int) -> Domain:
         """Instance k alone (its own iolet ids), for checks against a plain run."""
-        d, sl = self.domain, self.instances[k].sites
+        inst = self.instances[k]
+        d, sl = self.domain, inst.sites
         ids = d.iolet_id[sl].copy()
-        ids[d.site_type[sl] != 1] -= k
+        ids[d.site_type[sl] == 2] -= inst.inlets.start
+        ids[d.site_type[sl] == 3] -= inst.outlets.start
         return Domain(d.Q, d.coords[sl].copy(), d.site_type[sl].copy(), d.wall_mask[sl].copy(),
                       d.iolet_mask[sl].copy(), ids, d.cut[sl].copy(), d.normal[sl].copy(),
                       [], [])
 
 
-def stenosed_ensemble(count: int = 96, seed: int = 0, radius_range=(8.0, 40.0),
-                      voxel_size_m: float = 5e-6, period: float = 2000.0,
-                      steps: int = 4000) -> Ensemble:
-    """`count` seeded stenosed cylinders: radius ~U(radius_range), length
-    10 x radius, Gaussian stenosis of area reduction ~U(0.2, 0.7) and width
-    ~U(1, 3) x radius at a random axial position, tau ~U(0.6, 1.0), inlet
-    pressure 80 + A cos(2 pi t / period) mmHg with A ~U(0.5, 2) mmHg, outlet
-    80 mmHg (the reference pressure, density 1).  D3Q19 LBGK with BFL walls
-    and Nash iolets.  Each instance sits in its own aligned cube of 8^3
-    blocks, so that the Morton order keeps it contiguous."""
+def make_ensemble(parts: list, period: float, voxel_size_m: float = 5e-6,
+                  steps: int = 4000, name: str = "ensemble") -> Ensemble:
+    """An ensemble of independent vessels: parts = [(geometry, Instance
+    fields radius, length, severity, width, z0, tau, amplitude_mmHg)] with one
+    inlet and any number of outlets each.  Inlet pressure 80 + A cos(2 pi t /
+    period) mmHg, outlets 80 mmHg (the reference pressure, density 1), D3Q19
+    LBGK, BFL walls, Nash iolets.  Each vessel sits in its own aligned cube of
+    8^3 blocks, so that the Morton order keeps it contiguous."""
     import numpy as np
 
-    rng = np.random.default_rng(seed)
-    params = []
-    for _ in range(count):
-        R = float(rng.uniform(*radius_range))
-        length = int(round(10 * R))
-        sev = float(rng.uniform(0.2, 0.7))
-        width = float(rng.uniform(1.0, 3.0) * R)
-        z0 = float(rng.uniform(0.3, 0.7) * length)
-        tau = float(rng.uniform(0.6, 1.0))
-        amp = float(rng.uniform(0.5, 2.0))
-        params.append((R, length, sev, width, z0, tau, amp))
-    extent = max(max(2 * (int(np.ceil(p[0])) + 2), p[1] + 2) for p in params)
+    doms = [g.project(19) for g, *_ in parts]
+    extent = max(int((d.coords.max(0) - d.coords.min(0)).max()) + 1 for d in doms)
     side = 8
     while side < extent + 8:
         side *= 2
-    per_row = int(np.ceil(count ** (1 / 3)))
-    parts, instances, ins, outs = [], [], [], []
+    per_row = max(1, int(np.ceil(len(parts) ** (1 / 3))))
+    pieces, instances, ins, outs = [], [], [], []
     start = 0
-    for k, (R, length, sev, width, z0, tau, amp) in enumerate(params):
-        g = Geometry.stenosed_cylinder(R, length, sev, z0, width)
-        d = g.project(19)
+    n_in = n_out = 0
+    for k, ((g, R, length, sev, width, z0, tau, amp), d) in enumerate(zip(parts, doms)):
         cube = (k % per_row, (k // per_row) % per_row, k // (per_row * per_row))
         off = np.array([c * side - int(d.coords[:, a].min()) + 8 for a, c in enumerate(cube)],
                        np.int32)
-        coords = d.coords + off
         ids = d.iolet_id.copy()
-        ids[d.site_type != 1] += k
-        parts.append(Domain(19, coords, d.site_type, d.wall_mask, d.iolet_mask, ids, d.cut,
-                            d.normal, [], []))
+        ids[d.site_type == 2] += n_in
+        ids[d.site_type == 3] += n_out
+        pieces.append(Domain(19, d.coords + off, d.site_type, d.wall_mask, d.iolet_mask, ids,
+                             d.cut, d.normal, [], []))
         dt = (tau - 0.5) * L.Cs2 * voxel_size_m ** 2 * BLOOD_DENSITY / BLOOD_VISCOSITY
         units = L.UnitConverter(dt, voxel_size_m, BLOOD_DENSITY, 80.0)
         rho_amp = units.ConvertPressureDifferenceToLatticeUnits(amp) / L.Cs2
@@ -208,15 +199,43 @@ def stenosed_ensemble(count: int = 96, seed: int = 0, radius_range=(8.0, 40.0),
         for p in d.outlets:
             outs.append(L.InOutLetCosine(tuple(np.asarray(p.position) + off), tuple(p.normal),
                                          density_mean=1.0, density_amp=0.0, period=period))
-        instances.append(Instance(R, length, sev, width, z0, tau, amp, tuple(int(v) for v in off),
-                                  slice(start, start + d.n)))
+        instances.append(Instance(R, length, sev, width, z0, tau, amp,
+                                  tuple(int(v) for v in off), slice(start, start + d.n),
+                                  slice(n_in, n_in + len(d.inlets)),
+                                  slice(n_out, n_out + len(d.outlets))))
+        n_in += len(d.inlets)
+        n_out += len(d.outlets)
         start += d.n
-    cat = lambda f: np.concatenate([getattr(p, f) for p in parts])  # noqa: E731
+    cat = lambda f: np.concatenate([getattr(p, f) for p in pieces])  # noqa: E731
     dom = Domain(19, cat("coords"), cat("site_type"), cat("wall_mask"), cat("iolet_mask"),
                  cat("iolet_id"), cat("cut"), cat("normal"), [], [])
-    inst = np.concatenate([np.full(p.n, k, np.int32) for k, p in enumerate(parts)])
+    inst = np.concatenate([np.full(p.n, k, np.int32) for k, p in enumerate(pieces)])
     traits = L.Traits(L.D3Q19, L.LBGK, L.BouzidiFirdaousLallemand, L.NashZerothOrderPressure,
                       L.NashZerothOrderPressure)
-    return Ensemble(f"stenosed_ensemble_{count}_seed{seed}_D3Q19_LBGK_BFL_Nash", dom, traits,
-                    instances, inst, ins, outs, steps, period, voxel_size_m,
-                    dict(cube_side=side, sites=int(dom.n)))
+    return Ensemble(name, dom, traits, instances, inst, ins, outs, steps, period,
+                    voxel_size_m, dict(cube_side=side, sites=int(dom.n)))
+
+
+def stenosed_ensemble(count: int = 96, seed: int = 0, radius_range=(8.0, 40.0),
+                      voxel_size_m: float = 5e-6, period: float = 2000.0,
+                      steps: int = 4000) -> Ensemble:
+    """`count` seeded stenosed cylinders: radius ~U(radius_range), length
+    10 x radius, Gaussian stenosis of area reduction ~U(0.2, 0.7) and width
+    ~U(1, 3) x radius at a random axial position, tau ~U(0.6, 1.0), inlet
+    pressure amplitude ~U(0.5, 2) mmHg."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    parts = []
+    for _ in range(count):
+        R = float(rng.uniform(*radius_range))
+        length = int(round(10 * R))
+        sev = float(rng.uniform(0.2, 0.7))
+        width = float(rng.uniform(1.0, 3.0) * R)
+        z0 = float(rng.uniform(0.3, 0.7) * length)
+        tau = float(rng.uniform(0.6, 1.0))
+        amp = float(rng.uniform(0.5, 2.0))
+        g = Geometry.stenosed_cylinder(R, length, sev, z0, width)
+        parts.append((g, R, length, sev, width, z0, tau, amp))
+    return make_ensemble(parts, period, voxel_size_m, steps,
+                         f"stenosed_ensemble_{count}_seed{seed}_D3Q19_LBGK_BFL_Nash")

@@ -18,7 +18,7 @@ def ensemble():
 def _alone(e, k, storage):
     inst = e.instances[k]
     lbm = L.LBM(e.traits, e.instance_domain(k), L.LbmParameters(tau=inst.tau),
-                [e.inlets[k]], [e.outlets[k]], storage=storage, rho0=1.0)
+                e.inlets[inst.inlets], e.outlets[inst.outlets], storage=storage, rho0=1.0)
     lbm.SetInitialConditions(1.0)
     return lbm
 
