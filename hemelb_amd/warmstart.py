@@ -433,13 +433,16 @@ def train(net: "UNet", data, epochs: int, lr: float = 1e-3, seed: int = 0,
             _device_sync()
             net.forward(x, device=True)
             loss = net.backward(t, data["masks"][r], w, float(data["counts"][r]), device=True)
-            frac = done / max(total, 1)
+            elapsed = _time.perf_counter() - t0
+            frac = min(1.0, max(done / max(total, 1), elapsed / time_budget_s))
             net.adam(lr=lr * 0.5 * (1.0 + math.cos(math.pi * frac)))
             acc += loss
             done += 1
-            if _time.perf_counter() - t0 > time_budget_s:
+            if done % 500 == 0:
+                log(f"[warmstart] step {done}: loss {loss:.5f} ({elapsed:.0f}s)")
+            if elapsed > time_budget_s:
                 break
-        losses.append(acc / max(len(order), 1))
+        losses.append(acc / max(done - ep * R * phases, 1))
         log(f"[warmstart] epoch {ep}: mean loss {losses[-1]:.5f} "
             f"({_time.perf_counter() - t0:.0f}s)")
         if _time.perf_counter() - t0 > time_budget_s:
