@@ -104,6 +104,7 @@ _SIGS = {
     "hlb_instance_count": (C.c_int32, [_P]),
     "hlb_reduce_instances": (C.c_int, [_P, C.POINTER(hlb_instance_monitor)]),
     "hlb_velocity_change_instances": (C.c_int, [_P, _P, _P]),
+    "hlb_halo_unpack": (C.c_int, [_P, _P, C.c_int64, _P]),
     "hlb_set_launch_shape": (C.c_int, [_P, C.POINTER(hlb_launch_shape)]),
     "hlb_get_launch_shape": (C.c_int, [_P, C.POINTER(hlb_launch_shape)]),
     "hlb_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(hlb_launch_shape),

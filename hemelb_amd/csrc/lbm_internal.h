@@ -130,6 +130,10 @@ int launch_velocity_change_ranges(int64_t nchunks, int64_t np, const int64_t* ra
 int launch_velocity_change(int64_t n, int64_t np, const double* u,
                            const double* u_ref, double inv_ref, double* partials,
                            int nblocks, void* stream);
+// Halo unpack: state[slot_index[i]] = in[i] (storage conversion applied).
+int launch_scatter_pairs(int64_t count, int64_t np, const int64_t* slot_index,
+                         const double* in, int storage, double rho0, const double* w,
+                         void* state, void* stream);
 int launch_gather_pairs(int64_t count, int64_t np, const int64_t* slot_index,
                         const void* state, int storage, double rho0,
                         const double* w, double* out, void* stream);

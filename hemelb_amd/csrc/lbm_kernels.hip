@@ -1081,6 +1081,18 @@ __global__ void k_gather_pairs(int64_t count, int64_t np, const int64_t* slot,
   out[i] = v;
 }
 
+// Halo unpack: state[slot[i]] = in[i] (shifted storage stores in - w rho0).
+template <typename S>
+__global__ void k_scatter_pairs(int64_t count, int64_t np, const int64_t* slot,
+                                const double* in, double rho0, const double* w, S* state) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t sidx = slot[i];
+  double v = in[i];
+  if constexpr (kShifted<S>) v -= w[sidx / np] * rho0;
+  Store<S>::st(state + sidx, (typename Store<S>::T)v);
+}
+
 inline dim3 grid_for(int64_t n, int threads = 256) {
   return dim3((unsigned)((n + threads - 1) / threads));
 }
@@ -1279,6 +1291,23 @@ int launch_gather_pairs(int64_t count, int64_t np, const int64_t* slot_index,
     hipLaunchKernelGGL(k_gather_pairs<_Float16>, grid_for(count), dim3(256), 0,
                        (hipStream_t)stream, count, np, slot_index,
                        (const _Float16*)state, rho0, w, out);
+  return (int)hipGetLastError();
+}
+
+int launch_scatter_pairs(int64_t count, int64_t np, const int64_t* slot_index,
+                         const double* in, int storage, double rho0, const double* w,
+                         void* state, void* stream) {
+  if (storage == 0)
+    hipLaunchKernelGGL(k_scatter_pairs<double>, grid_for(count), dim3(256), 0,
+                       (hipStream_t)stream, count, np, slot_index, in, rho0, w,
+                       (double*)state);
+  else if (storage == 1)
+    hipLaunchKernelGGL(k_scatter_pairs<float>, grid_for(count), dim3(256), 0,
+                       (hipStream_t)stream, count, np, slot_index, in, rho0, w, (float*)state);
+  else
+    hipLaunchKernelGGL(k_scatter_pairs<_Float16>, grid_for(count), dim3(256), 0,
+                       (hipStream_t)stream, count, np, slot_index, in, rho0, w,
+                       (_Float16*)state);
   return (int)hipGetLastError();
 }
 

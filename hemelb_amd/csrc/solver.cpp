@@ -1406,6 +1406,36 @@ int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count, double* ho
   return HLB_OK;
 }
 
+int hlb_halo_unpack(hlb_solver* s, const int64_t* pairs, int64_t count,
+                    const double* host_buffer) {
+  if (!s || (count && (!pairs || !host_buffer))) return fail(HLB_E_INVALID, "null argument");
+  if (s->post_stream)
+    return fail(HLB_E_STATE, "no post-collision state yet (step at least once)");
+  if (count == 0) return HLB_OK;
+  std::vector<int64_t> slots(count);
+  for (int64_t k = 0; k < count; ++k) {
+    const int64_t site = pairs[2 * k], d = pairs[2 * k + 1];
+    if (site < 0 || site >= s->n || d < 0 || d >= s->Q)
+      return fail(HLB_E_INVALID, "halo pair out of range");
+    slots[k] = d * s->np + s->slot_of[s->internal[site]];
+  }
+  {
+    std::vector<int64_t> sorted(slots);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      return fail(HLB_E_INVALID, "a (site, direction) pair appears twice");
+  }
+  DevBuf<int64_t> d_slots;
+  DevBuf<double> d_in;
+  HIP_OR_FAIL(d_slots.upload(slots));
+  HIP_OR_FAIL(d_in.upload(std::vector<double>(host_buffer, host_buffer + count)));
+  if (hlb::launch_scatter_pairs(count, s->np, d_slots.p, d_in.p, s->p.storage, s->args.rho0,
+                                s->weights.p, s->state[s->cur], s->stream))
+    return fail(HLB_E_DEVICE, "halo unpack kernel launch failed");
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  return HLB_OK;
+}
+
 int64_t hlb_site_count(const hlb_solver* s) { return s ? s->n : -1; }
 int32_t hlb_lattice_q(const hlb_solver* s) { return s ? s->Q : -1; }
 int64_t hlb_device_bytes(const hlb_solver* s) { return s ? s->device_bytes() : -1; }

@@ -405,6 +405,13 @@ class LBM:
                                       pairs.shape[0], out.ctypes.data_as(C.c_void_p)))
         return out
 
+    def halo_unpack(self, pairs: np.ndarray, values: np.ndarray) -> None:
+        pairs = np.ascontiguousarray(pairs, np.int64)
+        values = np.ascontiguousarray(values, np.float64)
+        assert values.shape[0] == pairs.shape[0]
+        N.check(N.lib().hlb_halo_unpack(self._h, pairs.ctypes.data_as(C.c_void_p),
+                                        pairs.shape[0], values.ctypes.data_as(C.c_void_p)))
+
     def device_bytes(self) -> int:
         return int(N.lib().hlb_device_bytes(self._h))
 

@@ -239,11 +239,17 @@ int hlb_snapshot_velocity(hlb_solver* s);
 int hlb_velocity_change(hlb_solver* s, double reference_speed, double* out);
 
 /* Halo exchange for a subdomain (Code/geometry/FieldData.cc:27-49): pack the
- * post-collision distributions that leave through the given (site, direction)
- * pairs into a device buffer, or unpack received values into the ghost
- * slots.  `pairs` is int64[count][2] of (caller site index, direction). */
+ * pending post-collision distributions of the given (site, direction) pairs
+ * (the values the reference sends to the neighbouring rank) into a host
+ * buffer.  `pairs` is int64[count][2] of (caller site index, direction). */
 int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count,
                   double* host_buffer);
+/* The inverse of hlb_halo_pack: the values streaming in from a neighbouring
+ * subdomain replace the pending post-collision values of the given (site,
+ * direction) pairs, which must be distinct (the receive side of
+ * FieldData::SendAndReceive / CopyReceived, FieldData.cc:27-49). */
+int hlb_halo_unpack(hlb_solver* s, const int64_t* pairs, int64_t count,
+                    const double* host_buffer);
 
 /* Device timing of n steps with HIP events on the solver's stream
  * (milliseconds, launch gaps included). */
