@@ -118,6 +118,16 @@ int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
 int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
                   const double* rho, const double* u, double* partials,
                   int nblocks, void* stream);
+// Stress tensor [6][np] (xx xy xz yy yz zz), von Mises [np] and shear rate
+// [np] of every site of a binary64 SoA state (any output may be null); the
+// tau of site x is tau_site[x] when given, else tau.
+int launch_stress_fields(int Q, int64_t n, int64_t np, const double* soa, double tau,
+                         const double* tau_site, double* t6, double* vm, double* sr,
+                         void* stream);
+// Traction and tangential traction [nw][3] at the wall sites.
+int launch_traction(int Q, int64_t nw, int64_t np, const double* soa, const int64_t* site,
+                    const float* normal, const double* tau_of, double* tr, double* tg,
+                    void* stream);
 // Per-chunk statistics of sites [range[2c], range[2c+1]): partials[5c..]
 // = mass, min rho, max rho, max |u|, unstable; velocity change per chunk
 // with its own 1/reference speed.

@@ -1069,6 +1069,127 @@ __global__ __launch_bounds__(256) void k_wall_shear_stress(
   out[i] = sqrt(sq - nn * nn);
 }
 
+// The other stress outputs of the reference's property cache (Lattice.h):
+// f_neq = f - f_eq (SSE3 order), Pi_ab = sum_l f_neq_l c_la c_lb in direction
+// order.  Stress tensor (CalculateStressTensor, 630-645): Pi * (1 - 1/(2 tau))
+// plus (rho - 1) Cs2 on the diagonal, upper triangle xx xy xz yy yz zz;
+// von Mises stress (508-552); shear rate (728-753, 890-907).  The tau of
+// each site is tau_site[x] when given.
+template <int Q>
+__global__ __launch_bounds__(256) void k_stress_fields(
+    int64_t n, int64_t np, const double* __restrict__ soa, double tau,
+    const double* __restrict__ tau_site, double* __restrict__ t6,
+    double* __restrict__ vm, double* __restrict__ sr) {
+  using L = Lat<Q>;
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const double ts = tau_site ? tau_site[x] : tau;
+  double f[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    f[d] = soa[(int64_t)d * np + x];
+  });
+  double r, mx, my, mz;
+  moments<Q, double>(f, r, mx, my, mz);
+  const FeqCoef<double> fe(r, mx, my, mz);
+  double fneq[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    fneq[d] = f[d] - fe.template eq<Q, d>();
+  });
+  const double p00 = second_moment<Q, 0, 0>(fneq), p10 = second_moment<Q, 1, 0>(fneq),
+               p11 = second_moment<Q, 1, 1>(fneq), p20 = second_moment<Q, 2, 0>(fneq),
+               p21 = second_moment<Q, 2, 1>(fneq), p22 = second_moment<Q, 2, 2>(fneq);
+  if (t6) {
+    const double fac = 1 - 1 / (2 * ts);
+    const double pr = (r - 1) * kCs2;
+    t6[x] = p00 * fac + pr;
+    t6[np + x] = p10 * fac;
+    t6[2 * np + x] = p20 * fac;
+    t6[3 * np + x] = p11 * fac + pr;
+    t6[4 * np + x] = p21 * fac;
+    t6[5 * np + x] = p22 * fac + pr;
+  }
+  if (vm) {
+    // sums of f_neq (cx^2 - cy^2) etc. in direction order
+    double xx_yy = 0.0, yy_zz = 0.0, xx_zz = 0.0, xy = 0.0, xz = 0.0, yz = 0.0;
+    sfor<0, Q>([&](auto li) {
+      constexpr int l = decltype(li)::value;
+      constexpr int cx = L::c[l][0], cy = L::c[l][1], cz = L::c[l][2];
+      constexpr int a = cx * cx - cy * cy, b = cy * cy - cz * cz, c = cx * cx - cz * cz;
+      if constexpr (a != 0) xx_yy += fneq[l] * (double)a;
+      if constexpr (b != 0) yy_zz += fneq[l] * (double)b;
+      if constexpr (c != 0) xx_zz += fneq[l] * (double)c;
+      if constexpr (cx * cy != 0) xy += fneq[l] * (double)cx * (double)cy;
+      if constexpr (cx * cz != 0) xz += fneq[l] * (double)cx * (double)cz;
+      if constexpr (cy * cz != 0) yz += fneq[l] * (double)cy * (double)cz;
+    });
+    const double a = xx_yy * xx_yy + yy_zz * yy_zz + xx_zz * xx_zz;
+    const double b = xy * xy + xz * xz + yz * yz;
+    const double sp = (1.0 - 1.0 / (2.0 * ts)) / sqrt(2.0);
+    vm[x] = sp * sqrt(a + 6.0 * b);
+  }
+  if (sr) {
+    const double k = -1.0 / (2.0 * ts * r * kCs2);
+    const double e[6] = {p00 * k, p10 * k, p20 * k, p11 * k, p21 * k, p22 * k};
+    // rows: (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+    double acc = 0.0;
+    acc += e[0] * e[0];
+    acc += 2 * e[1] * e[1];
+    acc += 2 * e[2] * e[2];
+    acc += e[3] * e[3];
+    acc += 2 * e[4] * e[4];
+    acc += e[5] * e[5];
+    sr[x] = sqrt(2 * acc);
+  }
+}
+
+// Traction sigma n and its tangential projection t - n (t . n) at the wall
+// sites (CalculateTractionOnAPoint / CalculateTangentialProjectionTraction,
+// Lattice.h:560-602); tau_of[i] is the tau of wall site i.
+template <int Q>
+__global__ __launch_bounds__(256) void k_traction(
+    int64_t nw, int64_t np, const double* __restrict__ soa,
+    const int64_t* __restrict__ site, const float* __restrict__ nrm,
+    const double* __restrict__ tau_of, double* __restrict__ tr, double* __restrict__ tg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  const int64_t x = site[i];
+  double f[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    f[d] = soa[(int64_t)d * np + x];
+  });
+  double r, mx, my, mz;
+  moments<Q, double>(f, r, mx, my, mz);
+  const FeqCoef<double> fe(r, mx, my, mz);
+  double fneq[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    fneq[d] = f[d] - fe.template eq<Q, d>();
+  });
+  const double fac = 1 - 1 / (2 * tau_of[i]);
+  const double pr = (r - 1) * kCs2;
+  const double p00 = second_moment<Q, 0, 0>(fneq), p10 = second_moment<Q, 1, 0>(fneq),
+               p11 = second_moment<Q, 1, 1>(fneq), p20 = second_moment<Q, 2, 0>(fneq),
+               p21 = second_moment<Q, 2, 1>(fneq), p22 = second_moment<Q, 2, 2>(fneq);
+  const double sg[3][3] = {{p00 * fac + pr, p10 * fac, p20 * fac},
+                           {p10 * fac, p11 * fac + pr, p21 * fac},
+                           {p20 * fac, p21 * fac, p22 * fac + pr}};
+  const double n[3] = {(double)nrm[3 * i], (double)nrm[3 * i + 1], (double)nrm[3 * i + 2]};
+  double t[3];
+  for (int a = 0; a < 3; ++a) {
+    t[a] = 0.0;
+    for (int b = 0; b < 3; ++b) t[a] += sg[a][b] * n[b];
+  }
+  double tn = 0.0;
+  for (int a = 0; a < 3; ++a) tn = tn + t[a] * n[a];
+  for (int a = 0; a < 3; ++a) {
+    tr[3 * i + a] = t[a];
+    tg[3 * i + a] = t[a] - n[a] * tn;
+  }
+}
+
 template <typename S>
 __global__ void k_gather_pairs(int64_t count, int64_t np, const int64_t* slot,
                                const S* state, double rho0, const double* w,
@@ -1243,6 +1364,32 @@ int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
   else
     hipLaunchKernelGGL(k_reduce<19>, dim3(nblocks), dim3(256), 0,
                        (hipStream_t)stream, n, np, soa, rho, u, partials);
+  return (int)hipGetLastError();
+}
+
+int launch_stress_fields(int Q, int64_t n, int64_t np, const double* soa, double tau,
+                         const double* tau_site, double* t6, double* vm, double* sr,
+                         void* stream) {
+  if (n == 0) return 0;
+  if (Q == 15)
+    hipLaunchKernelGGL(k_stress_fields<15>, grid_for(n), dim3(256), 0, (hipStream_t)stream, n,
+                       np, soa, tau, tau_site, t6, vm, sr);
+  else
+    hipLaunchKernelGGL(k_stress_fields<19>, grid_for(n), dim3(256), 0, (hipStream_t)stream, n,
+                       np, soa, tau, tau_site, t6, vm, sr);
+  return (int)hipGetLastError();
+}
+
+int launch_traction(int Q, int64_t nw, int64_t np, const double* soa, const int64_t* site,
+                    const float* normal, const double* tau_of, double* tr, double* tg,
+                    void* stream) {
+  if (nw == 0) return 0;
+  if (Q == 15)
+    hipLaunchKernelGGL(k_traction<15>, grid_for(nw), dim3(256), 0, (hipStream_t)stream, nw, np,
+                       soa, site, normal, tau_of, tr, tg);
+  else
+    hipLaunchKernelGGL(k_traction<19>, grid_for(nw), dim3(256), 0, (hipStream_t)stream, nw, np,
+                       soa, site, normal, tau_of, tr, tg);
   return (int)hipGetLastError();
 }
 

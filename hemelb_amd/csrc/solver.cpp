@@ -263,8 +263,9 @@ struct hlb_solver {
   std::vector<Run> runs;
   DevBuf<double> blk_omega, blk_s_tau;
   DevBuf<float> blk_omega_f;
-  std::vector<double> wall_temp;
-  DevBuf<double> d_wall_temp;
+  std::vector<double> wall_temp, wall_tau;
+  std::vector<int32_t> site_inst;  // instance of each internal site (ensembles)
+  DevBuf<double> d_wall_temp, d_wall_tau, d_site_tau, d_t6, d_vm, d_sr, d_tr, d_tg;
   // wall sites (internal compact index) and their normals, for the wall
   // shear stress output
   bool has_normals = false;
@@ -799,6 +800,11 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
     const double tau = instance ? inst_tau[instance[s->perm[o]]] : P.tau;
     const double stress_parameter = (1.0 - 1.0 / (2.0 * tau)) / std::sqrt(2.0);
     s->wall_temp.push_back(stress_parameter * (-std::sqrt(2.0)));
+    s->wall_tau.push_back(tau);
+  }
+  if (instance) {
+    s->site_inst.resize(n);
+    for (int64_t o = 0; o < n; ++o) s->site_inst[o] = instance[s->perm[o]];
   }
 
   // Device set-up.
@@ -1093,6 +1099,86 @@ int hlb_get_wall_shear_stress(hlb_solver* s, double* stress) {
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   std::fill(stress, stress + s->n, std::numeric_limits<double>::max());
   for (int64_t k = 0; k < nw; ++k) stress[s->perm[s->wall_sites[k]]] = w[k];
+  return HLB_OK;
+}
+
+int hlb_get_stress(hlb_solver* s, double* tensor, double* von_mises, double* shear_rate,
+                   double* traction, double* tangential) {
+  if (!s) return fail(HLB_E_INVALID, "null solver");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  if ((traction || tangential) && !s->has_normals)
+    return fail(HLB_E_STATE, "wall normals were not given to hlb_create");
+  int r = s->post_stream_soa();
+  if (r) return r;
+  const int64_t npc = s->npc, n = s->n;
+  if (!s->site_inst.empty() && !s->d_site_tau.p) {
+    std::vector<double> t(n);
+    for (int64_t o = 0; o < n; ++o) t[o] = s->inst_tau[s->site_inst[o]];
+    HIP_OR_FAIL(s->d_site_tau.upload(t));
+  }
+  if (tensor && !s->d_t6.p) HIP_OR_FAIL(s->d_t6.alloc((size_t)6 * npc));
+  if (von_mises && !s->d_vm.p) HIP_OR_FAIL(s->d_vm.alloc((size_t)npc));
+  if (shear_rate && !s->d_sr.p) HIP_OR_FAIL(s->d_sr.alloc((size_t)npc));
+  if (tensor || von_mises || shear_rate) {
+    if (hlb::launch_stress_fields(s->Q, n, npc, s->s_soa.p, s->p.tau, s->d_site_tau.p,
+                                  tensor ? s->d_t6.p : nullptr, von_mises ? s->d_vm.p : nullptr,
+                                  shear_rate ? s->d_sr.p : nullptr, s->stream))
+      return fail(HLB_E_DEVICE, "stress kernel launch failed");
+  }
+  const int64_t nw = (int64_t)s->wall_sites.size();
+  std::vector<double> h6, hvm, hsr, htr, htg;
+  if (tensor) h6.resize(6 * npc);
+  if (von_mises) hvm.resize(npc);
+  if (shear_rate) hsr.resize(npc);
+  if ((traction || tangential) && nw) {
+    if (!s->d_wall_sites.p) {
+      HIP_OR_FAIL(s->d_wall_sites.upload(s->wall_sites));
+      HIP_OR_FAIL(s->d_wall_normals.upload(s->wall_normals));
+      HIP_OR_FAIL(s->d_wall_temp.upload(s->wall_temp));
+      HIP_OR_FAIL(s->d_wss.alloc((size_t)nw));
+    }
+    if (!s->d_wall_tau.p) {
+      HIP_OR_FAIL(s->d_wall_tau.upload(s->wall_tau));
+      HIP_OR_FAIL(s->d_tr.alloc((size_t)3 * nw));
+      HIP_OR_FAIL(s->d_tg.alloc((size_t)3 * nw));
+    }
+    if (hlb::launch_traction(s->Q, nw, npc, s->s_soa.p, s->d_wall_sites.p, s->d_wall_normals.p,
+                             s->d_wall_tau.p, s->d_tr.p, s->d_tg.p, s->stream))
+      return fail(HLB_E_DEVICE, "traction kernel launch failed");
+    htr.resize(3 * nw);
+    htg.resize(3 * nw);
+    HIP_OR_FAIL(hipMemcpyAsync(htr.data(), s->d_tr.p, sizeof(double) * 3 * nw,
+                               hipMemcpyDeviceToHost, s->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(htg.data(), s->d_tg.p, sizeof(double) * 3 * nw,
+                               hipMemcpyDeviceToHost, s->stream));
+  }
+  if (tensor)
+    HIP_OR_FAIL(hipMemcpyAsync(h6.data(), s->d_t6.p, sizeof(double) * 6 * npc,
+                               hipMemcpyDeviceToHost, s->stream));
+  if (von_mises)
+    HIP_OR_FAIL(hipMemcpyAsync(hvm.data(), s->d_vm.p, sizeof(double) * npc,
+                               hipMemcpyDeviceToHost, s->stream));
+  if (shear_rate)
+    HIP_OR_FAIL(hipMemcpyAsync(hsr.data(), s->d_sr.p, sizeof(double) * npc,
+                               hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  for (int64_t o = 0; o < n; ++o) {
+    const int64_t c = s->perm[o];
+    if (tensor)
+      for (int k = 0; k < 6; ++k) tensor[6 * c + k] = h6[(size_t)k * npc + o];
+    if (von_mises) von_mises[c] = hvm[o];
+    if (shear_rate) shear_rate[c] = hsr[o];
+  }
+  // traction is zero away from the walls (Common.h:107-121)
+  if (traction) std::fill(traction, traction + 3 * n, 0.0);
+  if (tangential) std::fill(tangential, tangential + 3 * n, 0.0);
+  for (int64_t k = 0; k < (int64_t)htr.size() / 3; ++k) {
+    const int64_t c = s->perm[s->wall_sites[k]];
+    for (int a = 0; a < 3; ++a) {
+      if (traction) traction[3 * c + a] = htr[3 * k + a];
+      if (tangential) tangential[3 * c + a] = htg[3 * k + a];
+    }
+  }
   return HLB_OK;
 }
 

@@ -344,6 +344,18 @@ class LBM:
         N.check(N.lib().hlb_get_wall_shear_stress(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
 
+    def stress(self) -> dict:
+        """Stress tensor [n,6] (xx xy xz yy yz zz), von Mises stress, shear
+        rate, traction and tangential traction [n,3] of fOld (lattice units)."""
+        n = self.domain.n
+        out = {"tensor": np.empty((n, 6)), "von_mises": np.empty(n),
+               "shear_rate": np.empty(n), "traction": np.empty((n, 3)),
+               "tangential_traction": np.empty((n, 3))}
+        p = lambda k: out[k].ctypes.data_as(C.c_void_p)  # noqa: E731
+        N.check(N.lib().hlb_get_stress(self._h, p("tensor"), p("von_mises"), p("shear_rate"),
+                                       p("traction"), p("tangential_traction")))
+        return out
+
     def monitor(self) -> dict:
         m = N.hlb_monitor()
         N.check(N.lib().hlb_reduce(self._h, C.byref(m)))

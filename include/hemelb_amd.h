@@ -182,6 +182,15 @@ int hlb_get_macroscopic(hlb_solver* s, double* density, double* velocity);
  * hlb_domain.wall_normal at create. */
 int hlb_get_wall_shear_stress(hlb_solver* s, double* stress);
 
+/* The other stress outputs of the property cache (Code/lb/streamers/
+ * Common.h:52-124) of fOld in the caller's order, each optional (NULL):
+ * stress tensor [n][6] (xx, xy, xz, yy, yz, zz; Lattice::CalculateStressTensor,
+ * Lattice.h:630-645), von Mises stress [n] (508-552), shear rate [n]
+ * (728-753), traction [n][3] and its tangential projection [n][3] at sites
+ * with wall links, zero elsewhere (560-602; needs wall normals). */
+int hlb_get_stress(hlb_solver* s, double* tensor, double* von_mises, double* shear_rate,
+                   double* traction, double* tangential);
+
 int hlb_reduce(hlb_solver* s, hlb_monitor* out);
 
 /* ---- Batched ensemble --------------------------------------------------

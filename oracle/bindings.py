@@ -65,6 +65,7 @@ _SIGS = {
     "orc_macro": (None, [_V, _V, _V]),
     "orc_wall_shear_stress": (_D, [C.c_int, _D, _V, _V]),
     "orc_wall_shear_stress_all": (None, [_V, _V, _V]),
+    "orc_stress_fields": (None, [C.c_int, _D, _V, _V, _V, _V, _V, _V, _V]),
 }
 _REF_SIGS = {
     "ref_lattice": (C.c_int, [C.c_int, _V, _V, _V, _V, _V]),
@@ -75,6 +76,7 @@ _REF_SIGS = {
     "ref_ipow_three_halves": (None, [_V]),
     "ref_pressure_to_lattice": (_D, [_D] * 5),
     "ref_wall_shear_stress": (_D, [C.c_int, _D, _V, _V]),
+    "ref_stress_fields": (C.c_int, [C.c_int, _D, _V, _V, _V, _V, _V, _V, _V]),
     "ref_pressure_diff_to_lattice": (_D, [_D] * 4),
     "ref_pressure_gradient_to_lattice": (_D, [_D] * 4),
 }
@@ -131,7 +133,9 @@ class OracleSim:
     def __init__(self, Q, kernel, wall, inlet_bc, outlet_bc, tau, coords, site_type,
                  wall_mask, iolet_mask, iolet_id, cut, inlets=(), outlets=()):
         self.Q = Q
+        self.tau = float(tau)
         self.n = int(coords.shape[0])
+        self._wall = np.ascontiguousarray(wall_mask, np.uint32)
         self._keep = [np.ascontiguousarray(coords, np.int32),
                       np.ascontiguousarray(site_type, np.uint32),
                       np.ascontiguousarray(wall_mask, np.uint32),
@@ -177,6 +181,26 @@ class OracleSim:
         u = np.empty((self.n, 3))
         lib().orc_macro(self._h, _p(rho), _p(u))
         return rho, u
+
+    def stress(self, normal):
+        """Per-site stress outputs of fOld (the oracle's orc_stress_fields)."""
+        n, Q = self.n, self.Q
+        f = self.get_f()
+        nrm = np.ascontiguousarray(normal, np.float32)
+        out = {"tensor": np.empty((n, 6)), "von_mises": np.empty(n),
+               "shear_rate": np.empty(n), "traction": np.zeros((n, 3)),
+               "tangential_traction": np.zeros((n, 3))}
+        vm, sr = np.zeros(1), np.zeros(1)
+        for i in range(n):
+            t6, tr, tg = np.zeros(6), np.zeros(3), np.zeros(3)
+            wall = self._wall[i] != 0
+            lib().orc_stress_fields(Q, self.tau, _p(np.ascontiguousarray(f[i])),
+                                    _p(nrm[i]) if wall else None, _p(t6), _p(tr), _p(tg),
+                                    _p(vm), _p(sr))
+            out["tensor"][i], out["von_mises"][i], out["shear_rate"][i] = t6, vm[0], sr[0]
+            if wall:
+                out["traction"][i], out["tangential_traction"][i] = tr, tg
+        return out
 
     def wall_shear_stress(self, normal):
         """Per-site WSS magnitude of fOld; float max at non-wall sites."""

@@ -195,6 +195,84 @@ double orc_wall_shear_stress(int Q, double tau, const double* f,
   return sqrt(sq - nn * nn);
 }
 
+/* Stress outputs of one site from f (fOld), following Lattice.h:
+ * stress tensor (CalculateStressTensor, 630-645: Pi(f_neq) * (1 - 1/(2 tau))
+ * plus (rho - 1) Cs2 on the diagonal; upper triangle xx, xy, xz, yy, yz, zz),
+ * traction sigma n and its tangential projection t - n (t . n)
+ * (CalculateTractionOnAPoint / CalculateTangentialProjectionTraction,
+ * 560-602; computed when normal != NULL), von Mises stress (508-552) and
+ * shear rate (728-753 with CalculateStrainRateTensorComponent, 890-907). */
+void orc_stress_fields(int Q, double tau, const double* f, const float* normal,
+                       double* tensor6, double* traction, double* tangential,
+                       double* von_mises, double* shear_rate) {
+  lattice_t L;
+  if (lattice_init(Q, &L)) return;
+  double rho, m[3], feq[19], fneq[19];
+  dens_mom(&L, f, &rho, m);
+  feq_all(&L, rho, m, feq);
+  for (int i = 0; i < Q; ++i) fneq[i] = f[i] - feq[i];
+  double pi[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b <= a; ++b) {
+      double acc = 0.0;
+      for (int l = 0; l < Q; ++l) acc += fneq[l] * L.cd[a][l] * L.cd[b][l];
+      pi[a][b] = acc;
+      pi[b][a] = acc;
+    }
+  double sig[3][3];
+  const double fac = 1 - 1 / (2 * tau);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) sig[a][b] = pi[a][b] * fac;
+  const double pressure = (rho - 1) * (1.0 / 3.0);
+  for (int a = 0; a < 3; ++a) sig[a][a] += pressure;
+  if (tensor6) {
+    tensor6[0] = sig[0][0]; tensor6[1] = sig[0][1]; tensor6[2] = sig[0][2];
+    tensor6[3] = sig[1][1]; tensor6[4] = sig[1][2]; tensor6[5] = sig[2][2];
+  }
+  if (normal) {
+    const double n[3] = {(double)normal[0], (double)normal[1], (double)normal[2]};
+    double t[3];
+    for (int a = 0; a < 3; ++a) {
+      t[a] = 0.0;
+      for (int b = 0; b < 3; ++b) t[a] += sig[a][b] * n[b];
+    }
+    if (traction)
+      for (int a = 0; a < 3; ++a) traction[a] = t[a];
+    double tn = 0.0;
+    for (int a = 0; a < 3; ++a) tn = tn + t[a] * n[a];
+    if (tangential)
+      for (int a = 0; a < 3; ++a) tangential[a] = t[a] - n[a] * tn;
+  }
+  if (von_mises) {
+    const double sp = (1.0 - 1.0 / (2.0 * tau)) / sqrt(2.0);
+    double xx_yy = 0.0, yy_zz = 0.0, xx_zz = 0.0, xy = 0.0, xz = 0.0, yz = 0.0;
+    for (int l = 0; l < Q; ++l) {
+      const int cx = L.c[l][0], cy = L.c[l][1], cz = L.c[l][2];
+      xx_yy += fneq[l] * (double)(cx * cx - cy * cy);
+      yy_zz += fneq[l] * (double)(cy * cy - cz * cz);
+      xx_zz += fneq[l] * (double)(cx * cx - cz * cz);
+      xy += fneq[l] * (double)cx * (double)cy;
+      xz += fneq[l] * (double)cx * (double)cz;
+      yz += fneq[l] * (double)cy * (double)cz;
+    }
+    const double a = xx_yy * xx_yy + yy_zz * yy_zz + xx_zz * xx_zz;
+    const double b = xy * xy + xz * xz + yz * yz;
+    *von_mises = sp * sqrt(a + 6.0 * b);
+  }
+  if (shear_rate) {
+    const double k = -1.0 / (2.0 * tau * rho * (1.0 / 3.0));
+    double sr = 0.0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = r; c < 3; ++c) {
+        double e = 0.0;
+        for (int l = 0; l < Q; ++l) e += fneq[l] * (double)(L.c[l][r] * L.c[l][c]);
+        e *= k;
+        sr += (r == c) ? e * e : 2 * e * e;
+      }
+    *shear_rate = sqrt(2 * sr);
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* Collision kernels                                                   */
 /* ------------------------------------------------------------------ */

@@ -68,6 +68,12 @@ int orc_mrt_basis(int Q, double* M, double* diag, double* s_for_tau, double tau)
  * wall normal `normal` (Lattice.h:652-689). */
 double orc_wall_shear_stress(int Q, double tau, const double* f,
                              const float* normal);
+/* Stress outputs of a site (stress tensor upper triangle, traction and its
+ * tangential projection for a wall normal, von Mises stress, shear rate);
+ * any output may be NULL, normal NULL skips the traction outputs. */
+void orc_stress_fields(int Q, double tau, const double* f, const float* normal,
+                       double* tensor6, double* traction, double* tangential,
+                       double* von_mises, double* shear_rate);
 void orc_bessel_j0(double re, double im, double tolsq, double* out);
 void orc_ipow_three_halves(double* out);
 double orc_cosine_density(const orc_iolet* io, uint64_t t0);

@@ -78,6 +78,35 @@ double wss(double tau, const double* f, const float* nrm) {
   return stress;
 }
 
+// The other stress outputs of Lattice.h from the same f_neq.
+template <class L>
+void stress_fields(double tau, const double* f, const float* nrm, double* t6, double* tr,
+                   double* tg, double* vm, double* sr) {
+  std::array<distribn_t, L::NUMVECTORS> fa, fe, fneq;
+  std::memcpy(fa.data(), f, sizeof(double) * L::NUMVECTORS);
+  distribn_t density;
+  util::Vector3D<distribn_t> mom, vel;
+  L::CalculateDensityMomentumFEq(typename L::const_span(fa), density, mom, vel,
+                                 typename L::mut_span(fe));
+  for (unsigned i = 0; i < L::NUMVECTORS; ++i) fneq[i] = fa[i] - fe[i];
+  typename L::const_span span(fneq);
+  util::Matrix3D sigma;
+  L::CalculateStressTensor(density, tau, span, sigma);
+  t6[0] = sigma[0][0]; t6[1] = sigma[0][1]; t6[2] = sigma[0][2];
+  t6[3] = sigma[1][1]; t6[4] = sigma[1][2]; t6[5] = sigma[2][2];
+  const util::Vector3D<double> n(util::Vector3D<float>(nrm[0], nrm[1], nrm[2]));
+  util::Vector3D<LatticeStress> t, g;
+  L::CalculateTractionOnAPoint(density, tau, span, n, t);
+  L::CalculateTangentialProjectionTraction(density, tau, span, n, g);
+  for (int a = 0; a < 3; ++a) {
+    tr[a] = t[a];
+    tg[a] = g[a];
+  }
+  const double stress_parameter = (1.0 - 1.0 / (2.0 * tau)) / std::sqrt(2.0);
+  L::CalculateVonMisesStress(span, *vm, stress_parameter);
+  *sr = L::CalculateShearRate(tau, span, density);
+}
+
 template <class L>
 void lattice(int* cx, int* cy, int* cz, double* w, int* inv) {
   for (unsigned i = 0; i < L::NUMVECTORS; ++i) {
@@ -117,6 +146,15 @@ double ref_wall_shear_stress(int Q, double tau, const double* f, const float* no
   if (Q == 15) return wss<lb::D3Q15>(tau, f, normal);
   if (Q == 19) return wss<lb::D3Q19>(tau, f, normal);
   return -1.0;
+}
+
+int ref_stress_fields(int Q, double tau, const double* f, const float* normal, double* t6,
+                      double* traction, double* tangential, double* von_mises,
+                      double* shear_rate) {
+  if (Q == 15) stress_fields<lb::D3Q15>(tau, f, normal, t6, traction, tangential, von_mises, shear_rate);
+  else if (Q == 19) stress_fields<lb::D3Q19>(tau, f, normal, t6, traction, tangential, von_mises, shear_rate);
+  else return -1;
+  return 0;
 }
 
 int ref_mrt_basis(int Q, double* M, double* diag, double* s, double tau) {

@@ -198,3 +198,16 @@ def test_halo_unpack_restores_received_values(layout, storage):
     np.testing.assert_array_equal(a.get_distributions(), b.get_distributions())
     with pytest.raises(Exception, match="twice"):
         b.halo_unpack(np.concatenate([pairs[:2], pairs[:1]]), sent[:3])
+
+
+@pytest.mark.parametrize("label,make", [
+    ("fourcube-mrt-bfl", lambda: K.four_cube_case("MRT", "BFL")),
+    ("cyl19-lbgk-bfl-ladd-wom",
+     lambda: K.cylinder_case(6.0, 24, 19, "LBGK", "BFL", "womersley", tau=0.7))])
+def test_stress_outputs_match_oracle(label, make):
+    case = make()
+    orc, gpu = _run_and_compare(case, 15, label="stress " + label)
+    want = orc.stress(case.domain.normal)
+    got = gpu.stress()
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)

@@ -503,3 +503,51 @@ def test_wall_shear_stress_of_a_simple_shear_flow(Q):
         got = O.lib().orc_wall_shear_stress(Q, tau, P(f), P(np.float32([0, 1, 0])))
         want = rho * (tau - 0.5) / 3.0 * abs(gamma)
         assert abs(got - want) <= 1e-12 * want
+
+
+def _stress(lib_fn, Q, tau, f, n):
+    t6, tr, tg = np.zeros(6), np.zeros(3), np.zeros(3)
+    vm, sr = np.zeros(1), np.zeros(1)
+    lib_fn(Q, tau, P(f), P(n), P(t6), P(tr), P(tg), P(vm), P(sr))
+    return t6, tr, tg, vm[0], sr[0]
+
+
+@needs_ref
+@pytest.mark.parametrize("Q", [15, 19])
+def test_stress_outputs_bitwise_equal_reference(Q):
+    """Stress tensor, traction, tangential traction, von Mises stress and
+    shear rate (Lattice.h:508-907) against the reference's own functions."""
+    rng = np.random.default_rng(60 + Q)
+    for k in range(1500):
+        f, n, tau = _wss_inputs(Q, k, rng)
+        a = _stress(O.lib().orc_stress_fields, Q, tau, f, n)
+        b = _stress(O.ref().ref_stress_fields, Q, tau, f, n)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("Q", [15, 19])
+def test_stress_outputs_of_a_simple_shear_flow(Q):
+    """Chapman-Enskog f_neq of u_x = gamma*y: shear rate |gamma|;
+    CalculateStressTensor adds +(1 - 1/(2 tau)) Pi_neq (Lattice.h:630-645, the
+    opposite sign of its comment), so sigma_xy = (1 - 1/(2 tau)) *
+    (-tau rho cs^2 gamma) = -rho nu gamma; traction on a y wall
+    (sigma_xy, p', 0) with p' = (rho - 1) cs^2; von Mises sqrt(6) times the
+    stress parameter times |Pi_xy|."""
+    c, w, _ = _lattice(Q)
+    for tau, rho, gamma in [(0.8, 1.02, 1e-3), (0.6, 0.97, -4e-4)]:
+        feq = np.zeros(Q)
+        O.lib().orc_feq(Q, rho, P(np.zeros(3)), P(feq))
+        f = feq - tau * w * rho * 3.0 * c[:, 0] * c[:, 1] * gamma
+        t6, tr, tg, vm, sr = _stress(O.lib().orc_stress_fields, Q, tau, f,
+                                     np.float32([0, 1, 0]))
+        nu = (tau - 0.5) / 3.0
+        assert abs(sr - abs(gamma)) <= 1e-12 * abs(gamma)
+        assert abs(t6[1] + rho * nu * gamma) <= 1e-12 * abs(rho * nu * gamma)
+        pr = (rho - 1) / 3.0
+        np.testing.assert_allclose([t6[0], t6[3], t6[5]], [pr] * 3, atol=1e-15)
+        np.testing.assert_allclose(tr, [t6[1], pr, 0.0], atol=1e-15)
+        np.testing.assert_allclose(tg, [t6[1], 0.0, 0.0], atol=1e-15)
+        pi_xy = -tau * rho * gamma / 3.0
+        sp = (1 - 1 / (2 * tau)) / np.sqrt(2)
+        assert abs(vm - sp * np.sqrt(6.0) * abs(pi_xy)) <= 1e-12 * abs(vm)
