@@ -113,6 +113,11 @@ def main() -> int:
     ap.add_argument("--storage", choices=sorted(STORE_BYTES), default="fp64")
     ap.add_argument("--layout", choices=["blocks", "pull_table", "compact_blocks"],
                     default="blocks")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="workgroup size of the step kernel (0: the solver's default)")
+    ap.add_argument("--xcd-remap", type=int, default=-1,
+                    help="workgroup-to-XCD remap (-1: the solver's default; 0 hardware "
+                         "order, 1 one range per XCD, K>=2 runs of K workgroups per XCD)")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
@@ -142,6 +147,15 @@ def main() -> int:
     lbm = work.solver(storage=args.storage, layout=args.layout)
     lbm.synchronize()
     setup_s = time.time() - t0
+    launch = None
+    if args.layout != "pull_table":
+        if args.threads or args.xcd_remap >= 0:
+            sh = lbm.launch_shape
+            lbm.set_launch_shape(threads=args.threads or sh["threads"], planes=sh["planes"],
+                                 groups=sh["groups"],
+                                 xcd_remap=sh["xcd_remap"] if args.xcd_remap < 0
+                                 else args.xcd_remap)
+        launch = lbm.launch_shape
     log(f"[bench] solver set-up {setup_s:.1f}s, device memory "
         f"{lbm.device_bytes() / 2**30:.2f} GiB")
     lbm.step(args.warmup)
@@ -164,7 +178,7 @@ def main() -> int:
     if os.path.exists(prof):
         try:
             rec = json.load(open(prof)).get(f"tree5_{args.storage}")
-            if rec and rec.get("sites") == n:
+            if rec and rec.get("sites") == n and rec.get("launch") == launch:
                 roof["traffic"] = rec["bytes_per_launch"]
         except (OSError, ValueError):
             pass
@@ -178,6 +192,8 @@ def main() -> int:
             if st == args.storage:
                 continue
             other = work.solver(storage=st, layout=args.layout)
+            if launch:
+                other.set_launch_shape(**launch)
             f_ms, f_mlups = timed_rate(other, n, args.form_steps, args.warmup)
             f_ach = 2 * 19 * STORE_BYTES[st] * n / (f_ms * 1e-3) / 1e9
             forms[st] = {"value": round(f_mlups, 2), "unit": "MLUPS",
@@ -202,7 +218,7 @@ def main() -> int:
         "config": {"workload": "vessel_tree_5gen_R40_D3Q19_LBGK_BFL_LaddWomersley_Nash",
                    "fluid_sites": n, "lattice": "D3Q19", "kernel": "LBGK", "tau": tau,
                    "walls": "BFL", "inlet": "Ladd/Womersley", "outlets": len(outlets),
-                   "storage": args.storage, "layout": args.layout,
+                   "storage": args.storage, "layout": args.layout, "launch": launch,
                    "setup_s": round(setup_s, 2),
                    "bytes_alg_per_site": b_alg,
                    **{k: round(v, 6) for k, v in work.meta.items()}},

@@ -370,6 +370,21 @@ __device__ __forceinline__ uint32_t logical_workgroup() {
   return start + k;
 }
 
+// Logical workgroup of the step kernels.  remap 0: the hardware order
+// (consecutive workgroups on different XCDs); 1: one contiguous range per
+// XCD; run >= 2: XCD x takes runs of `run` consecutive logical workgroups
+// and the eight XCDs work on eight adjacent runs at a time, so the planes
+// of a block and of its Morton neighbours stay in one XCD's L2 while the
+// whole chip sweeps one region.  The grid is a multiple of 8 * run (the
+// launcher pads it), which makes the map a bijection.
+__device__ __forceinline__ uint32_t step_workgroup(uint32_t remap) {
+  if (remap == 0) return blockIdx.x;
+  if (remap == 1) return logical_workgroup();
+  const uint32_t b = blockIdx.x, x = b & 7u, j = b >> 3;
+  const uint32_t r = j / remap, o = j - r * remap;
+  return (r * 8u + x) * remap + o;
+}
+
 // Compact layout: sites packed in Morton-block order.  The 32-bit pull
 // table holds, per direction e, the offset (inside array e) of the value
 // that streams into slot e: the source site for a fluid link, or the site's
@@ -748,7 +763,7 @@ __global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
                                               S* __restrict__ fout) {
   SiteRef r;
   if constexpr (kWaveLayout<Lay>) {
-    const uint32_t g = A.xcd_remap ? logical_workgroup() : blockIdx.x;
+    const uint32_t g = step_workgroup(A.xcd_remap);
     const uint32_t w = __builtin_amdgcn_readfirstlane(g * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (!Lay::locate_wave(D, w, r)) return;
   } else {
@@ -767,7 +782,7 @@ __global__ __launch_bounds__(256) void k_step_loop(DevDomain D, StepArgs A,
                                                    const S* __restrict__ fin,
                                                    S* __restrict__ fout) {
   const uint32_t wpg = blockDim.x >> 6;
-  const uint32_t g = A.xcd_remap ? logical_workgroup() : blockIdx.x;
+  const uint32_t g = step_workgroup(A.xcd_remap);
   const uint32_t gw = __builtin_amdgcn_readfirstlane(g * wpg + (threadIdx.x >> 6));
   const uint32_t stride = gridDim.x * wpg * A.planes;
   for (uint32_t base = gw * A.planes; base < D.nwaves; base += stride) {
@@ -1272,9 +1287,16 @@ int launch_step(const DevDomain& d, const StepArgs& a, const void* fin, void* fo
       const uint32_t wpg = a.threads / 64u;
       const uint64_t need = ((uint64_t)d.nwaves + (uint64_t)wpg * a.planes - 1) /
                             ((uint64_t)wpg * a.planes);
-      const uint64_t groups = a.groups ? std::min<uint64_t>(a.groups, need) : need;
+      uint64_t groups = a.groups ? std::min<uint64_t>(a.groups, need) : need;
+      const bool one_pass = a.planes == 1 && groups == need;
+      if (a.xcd_remap >= 2) {
+        // runs of a.xcd_remap workgroups per XCD: pad the grid to whole
+        // groups of 8 runs; padded workgroups see planes >= nwaves and exit
+        const uint64_t m = 8ull * a.xcd_remap;
+        groups = (groups + m - 1) / m * m;
+      }
       const dim3 grid((unsigned)std::max<uint64_t>(groups, 1)), block(a.threads);
-      if (a.planes == 1 && groups == need)
+      if (one_pass)
         hipLaunchKernelGGL((k_step<decltype(q)::value, decltype(k)::value, S, Lay>), grid,
                            block, 0, (hipStream_t)stream, d, a, (const S*)fin, (S*)fout);
       else

@@ -1371,6 +1371,8 @@ static int check_shape(const hlb_solver* s, const hlb_launch_shape& sh) {
     return fail(HLB_E_INVALID, "threads per workgroup must be 64, 128 or 256");
   if (sh.planes < 1 || sh.planes > 8) return fail(HLB_E_INVALID, "planes per wave must be 1..8");
   if (sh.groups < 0) return fail(HLB_E_INVALID, "workgroup count must not be negative");
+  if (sh.xcd_remap < 0 || sh.xcd_remap > 4096)
+    return fail(HLB_E_INVALID, "xcd_remap must be in 0..4096");
   return HLB_OK;
 }
 
@@ -1381,7 +1383,7 @@ int hlb_set_launch_shape(hlb_solver* s, const hlb_launch_shape* shape) {
   s->args.threads = (uint32_t)shape->threads;
   s->args.planes = (uint32_t)shape->planes;
   s->args.groups = (uint32_t)shape->groups;
-  s->args.xcd_remap = shape->xcd_remap ? 1u : 0u;
+  s->args.xcd_remap = (uint32_t)shape->xcd_remap;
   return HLB_OK;
 }
 
@@ -1423,6 +1425,9 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* b
   std::vector<hlb_launch_shape> cand;
   for (int th : {256, 128, 64})
     for (int xcd : {1, 0}) cand.push_back({th, 1, 0, xcd});
+  for (int run : {2, 16, 64}) cand.push_back({256, 1, 0, run});
+  cand.push_back({128, 1, 0, 4});
+  cand.push_back({128, 1, 0, 8});
   cand.push_back({256, 2, 0, 1});
   cand.push_back({128, 2, 0, 1});
   for (int th : {256, 128}) cand.push_back({th, 1, cus * 24 * 64 / th, 1});

@@ -226,7 +226,10 @@ int hlb_velocity_change_instances(hlb_solver* s, const double* reference_speed,
 /* Launch shape of the step kernel (block layouts): threads per workgroup
  * (64, 128, 256), x planes (64-site tiles) per wave, workgroups in the grid
  * (0: one pass over the domain; a positive count gives a persistent grid
- * that strides over it) and the XCD-contiguous workgroup remap (0/1).
+ * that strides over it) and the workgroup-to-XCD remap: 0 the hardware's
+ * round-robin order, 1 one contiguous range of the domain per XCD, K in
+ * 2..4096 runs of K consecutive workgroups per XCD with the eight XCDs on
+ * adjacent runs (the grid is padded to a multiple of 8K).
  * hlb_autotune times a set of shapes for `steps` steps each on the current
  * state, restores the state and time, and keeps the fastest. */
 typedef struct {

@@ -65,8 +65,15 @@ def test_launch_shapes_agree_bitwise(ensemble):
     ref = e.solver(storage="fp32")
     ref.step(12)
     want = ref.get_distributions()
-    shapes = [dict(threads=64), dict(threads=128, xcd_remap=False), dict(planes=2),
-              dict(threads=128, planes=2), dict(groups=7), dict(planes=3, groups=5)]
+    shapes = [dict(threads=64, xcd_remap=1), dict(threads=128), dict(planes=2, xcd_remap=1),
+              dict(threads=128, planes=2), dict(groups=7, xcd_remap=1),
+              dict(planes=3, groups=5), dict(xcd_remap=2), dict(xcd_remap=16),
+              dict(threads=128, xcd_remap=64), dict(groups=9, xcd_remap=4),
+              dict(planes=2, xcd_remap=8), dict(threads=64, xcd_remap=3),
+              dict(groups=11, xcd_remap=5)]
+    for bad in (dict(xcd_remap=4097), dict(xcd_remap=-1)):
+        with pytest.raises(Exception):
+            ref.set_launch_shape(**bad)
     for sh in shapes:
         lbm = e.solver(storage="fp32")
         lbm.set_launch_shape(**sh)
