@@ -95,6 +95,7 @@ _SIGS = {
     "hlb_step": (C.c_int, [_P, C.c_int64]),
     "hlb_get_distributions": (C.c_int, [_P, _P]),
     "hlb_set_distributions": (C.c_int, [_P, _P]),
+    "hlb_set_equilibrium_fields": (C.c_int, [_P, _P, _P, C.c_int32]),
     "hlb_get_macroscopic": (C.c_int, [_P, _P, _P]),
     "hlb_get_wall_shear_stress": (C.c_int, [_P, _P]),
     "hlb_get_stress": (C.c_int, [_P, _P, _P, _P, _P, _P]),

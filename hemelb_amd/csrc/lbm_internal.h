@@ -28,12 +28,13 @@ struct DevDomain {
   const uint64_t* wave_bnd = nullptr;    // [nwaves] lane has a boundary record
   const uint32_t* wave_rec = nullptr;    // [nwaves] first boundary record
   const uint32_t* wave_cbase = nullptr;  // [nwaves] compact index of first fluid lane
-  const uint32_t* slot_cut = nullptr;    // [np] wall|iolet link mask per slot
   // Pull table (layout 0): src[(d-1)*np + x] is the offset inside array d
   // of the value streaming into slot d of site x: the site at x - c_d, or
   // x + (inv(d) - d) * np (the site's own inverse slot) for a cut link.
   const int32_t* src = nullptr;
-  // 0 for a site with no wall/iolet link, else its boundary record + 1.
+  // Pull-table layout: 0 for a site with no wall/iolet link, else its
+  // boundary record + 1.  Block layouts find the record from the wave's
+  // boundary mask.
   const uint32_t* binfo = nullptr;
   int64_t nb = 0;                  // boundary records
   const uint32_t* b_wall = nullptr;   // [nb] wall link mask
@@ -108,6 +109,10 @@ int launch_soa_to_aos(int Q, int64_t n, int64_t np, const double* soa,
                       const int64_t* perm, double* aos, void* stream);
 int launch_import(const DevDomain& d, const StepArgs& a, const double* aos,
                   const int64_t* perm, void* state, void* stream);
+// Equilibrium of caller-order fields rho[n], momentum[n][3] into the state.
+int launch_import_equilibrium(const DevDomain& d, const StepArgs& a, const double* rho,
+                              const double* mom, const int64_t* perm, void* state,
+                              void* stream);
 // Moments of a binary64 SoA state: rho[np], u[3][np] (internal order).
 int launch_moments(int Q, int64_t n, int64_t np, const double* soa, double* rho,
                    double* u, void* stream);

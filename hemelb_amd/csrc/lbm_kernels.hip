@@ -412,7 +412,7 @@ struct SparseLayout {
 // z fastest); a wave covers one x plane of one block.  Neighbours come from
 // slot arithmetic and the block's 27-entry neighbour table, which each wave
 // loads once; per-wave 64-bit masks say which lanes are fluid and which carry
-// a boundary record, and a per-slot mask marks the cut links.
+// a boundary record, whose wall and iolet masks mark the cut links.
 struct BlockLayout {
   static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
     const int64_t slot = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
@@ -435,9 +435,10 @@ struct BlockLayout {
       constexpr int i = decltype(ii)::value;
       r.tb[i] = nbt[i];
     });
-    // Only waves holding a boundary site need the per-slot cut masks.
-    r.cut = bm ? D.slot_cut[r.x] : 0u;
+    // A lane's cut-link mask comes from its boundary record (the gather
+    // reads the same two words again for the wall and iolet formulas).
     r.rec = rb + (uint32_t)__popcll(bm & ((1ull << r.lane) - 1ull));
+    r.cut = ((bm >> r.lane) & 1ull) ? (D.b_wall[r.rec] | D.b_iolet[r.rec]) : 0u;
     return true;
   }
   template <int Q, int E>
@@ -511,8 +512,8 @@ struct CompactBlockLayout {
       constexpr int i = decltype(ii)::value;
       r.tb[i] = nbt[i];
     });
-    r.cut = bm ? D.slot_cut[r.x] : 0u;
     r.rec = rb + (uint32_t)__popcll(bm & ((1ull << r.lane) - 1ull));
+    r.cut = ((bm >> r.lane) & 1ull) ? (D.b_wall[r.rec] | D.b_iolet[r.rec]) : 0u;
     return true;
   }
   // Fluid mask and compact base of x plane px of block nb (nb < 0: absent,
@@ -854,6 +855,26 @@ __global__ __launch_bounds__(256) void k_import(DevDomain D, const double* aos,
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
     double v = aos[c * Q + d];
+    if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
+    Store<S>::st(state + (int64_t)d * D.np + r.x, (typename Store<S>::T)v);
+  });
+}
+
+// Equilibrium of a per-site (density, momentum) field, caller's site order
+// (perm maps the layout's compact index to it), with the collision's own
+// Lattice::CalculateFeq arithmetic.
+template <int Q, typename S, class Lay>
+__global__ __launch_bounds__(256) void k_import_equilibrium(DevDomain D, const double* rho,
+                                                            const double* mom,
+                                                            const int64_t* perm, S* state,
+                                                            double rho0) {
+  SiteRef r;
+  if (!Lay::locate(D, r)) return;
+  const int64_t c = perm[Lay::compact(D, r)];
+  const FeqCoef<double> fe(rho[c], mom[3 * c], mom[3 * c + 1], mom[3 * c + 2]);
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    double v = fe.template eq<Q, d>();
     if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
     Store<S>::st(state + (int64_t)d * D.np + r.x, (typename Store<S>::T)v);
   });
@@ -1342,6 +1363,19 @@ int launch_import(const DevDomain& d, const StepArgs& a, const double* aos,
     using Lay = std::remove_pointer_t<decltype(lay)>;
     hipLaunchKernelGGL((k_import<decltype(q)::value, S, Lay>), grid_for(thread_count(d)),
                        dim3(256), 0, (hipStream_t)stream, d, aos, perm, (S*)state, a.rho0);
+    return (int)hipGetLastError();
+  });
+}
+
+int launch_import_equilibrium(const DevDomain& d, const StepArgs& a, const double* rho,
+                              const double* mom, const int64_t* perm, void* state,
+                              void* stream) {
+  return dispatch(d, a, [&](auto q, auto, auto* s, auto* lay) {
+    using S = std::remove_pointer_t<decltype(s)>;
+    using Lay = std::remove_pointer_t<decltype(lay)>;
+    hipLaunchKernelGGL((k_import_equilibrium<decltype(q)::value, S, Lay>),
+                       grid_for(thread_count(d)), dim3(256), 0, (hipStream_t)stream, d, rho,
+                       mom, perm, (S*)state, a.rho0);
     return (int)hipGetLastError();
   });
 }

@@ -231,7 +231,7 @@ struct hlb_solver {
   int n_inlets = 0;
   // device data
   DevBuf<int32_t> src;
-  DevBuf<uint32_t> binfo, b_wall, b_iolet, slot_cut;
+  DevBuf<uint32_t> binfo, b_wall, b_iolet;
   DevBuf<int32_t> b_io, io_iolet, io_bc, io_site, iolet_kind;
   DevBuf<float> b_q, iolet_nf;
   DevBuf<double> io_stash, io_ladd, iolet_n, weights;
@@ -298,7 +298,7 @@ struct hlb_solver {
          b_io.bytes() + io_iolet.bytes() + io_bc.bytes() + io_site.bytes() +
          iolet_kind.bytes() + b_q.bytes() + iolet_nf.bytes() + io_stash.bytes() +
          io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
-         bnbr.bytes() + slot_cut.bytes() + wave_fluid.bytes() + wave_bnd.bytes() +
+         bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() +
          wave_rec.bytes() +
          wave_cbase.bytes() +
          s_soa.bytes() + s_aos.bytes() + rho.bytes() + u.bytes() + u_snap.bytes();
@@ -582,14 +582,12 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
 
   // Pull table, boundary and iolet records.
   std::vector<int32_t> h_src;
-  std::vector<uint32_t> h_binfo, h_cut;
+  std::vector<uint32_t> h_binfo;
   if (!blocks_layout) {
     if (np >= ((int64_t)1 << 30))
       return fail(HLB_E_INVALID, "too many sites for the pull-table layout");
     h_src.assign((size_t)(Q - 1) * np, 0);
     h_binfo.assign(np, 0);
-  } else {
-    h_cut.assign(np, 0);
   }
   std::vector<uint32_t> h_bwall, h_biolet;
   std::vector<int32_t> h_bio, h_io_iolet, h_io_bc, h_io_site;
@@ -626,7 +624,6 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
       if (!blocks_layout)
         h_src[(size_t)(e - 1) * np + o] = own ? (int32_t)(o + (int64_t)(dp - e) * np) : (int32_t)j;
     }
-    if (blocks_layout) h_cut[s->slot_of[o]] = wm | im;
     if (wm | im) {
       const uint32_t rec = (uint32_t)h_bwall.size();
       if (!blocks_layout) h_binfo[o] = rec + 1;
@@ -817,7 +814,6 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   HIP_OR_FAIL(s->src.upload(h_src));
   HIP_OR_FAIL(s->binfo.upload(h_binfo));
   HIP_OR_FAIL(s->bnbr.upload(h_bnbr));
-  HIP_OR_FAIL(s->slot_cut.upload(h_cut));
   HIP_OR_FAIL(s->wave_fluid.upload(h_wf));
   HIP_OR_FAIL(s->wave_bnd.upload(h_wb));
   HIP_OR_FAIL(s->wave_rec.upload(h_wr));
@@ -873,7 +869,6 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   D.wave_bnd = s->wave_bnd.p;
   D.wave_rec = s->wave_rec.p;
   D.wave_cbase = s->wave_cbase.p;
-  D.slot_cut = s->slot_cut.p;
   D.src = s->src.p;
   D.binfo = s->binfo.p;
   D.nb = (int64_t)h_bwall.size();
@@ -1034,6 +1029,30 @@ int hlb_set_distributions(hlb_solver* s, const double* f_in) {
   if (hlb::launch_import(s->dom, s->args, s->s_aos.p, s->d_perm.p, s->state[s->cur],
                          s->stream))
     return fail(HLB_E_DEVICE, "import kernel launch failed");
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  s->post_stream = true;
+  s->initialised = true;
+  return HLB_OK;
+}
+
+int hlb_set_equilibrium_fields(hlb_solver* s, const double* density, const double* momentum,
+                               int32_t device) {
+  if (!s || !density || !momentum) return fail(HLB_E_INVALID, "null argument");
+  const double* rho = density;
+  const double* mom = momentum;
+  DevBuf<double> hr, hm;
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  if (!device) {
+    HIP_OR_FAIL(hr.alloc((size_t)s->n));
+    HIP_OR_FAIL(hm.alloc((size_t)3 * s->n));
+    HIP_OR_FAIL(hipMemcpy(hr.p, density, sizeof(double) * s->n, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(hm.p, momentum, sizeof(double) * 3 * s->n, hipMemcpyHostToDevice));
+    rho = hr.p;
+    mom = hm.p;
+  }
+  if (hlb::launch_import_equilibrium(s->dom, s->args, rho, mom, s->d_perm.p,
+                                     s->state[s->cur], s->stream))
+    return fail(HLB_E_DEVICE, "equilibrium import kernel launch failed");
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   s->post_stream = true;
   s->initialised = true;

@@ -330,6 +330,32 @@ class LBM:
         assert f.shape == (self.domain.n, self.Q)
         N.check(N.lib().hlb_set_distributions(self._h, f.ctypes.data_as(C.c_void_p)))
 
+    def set_equilibrium_fields(self, density, momentum) -> None:
+        """fOld = feq(density, momentum) per site (caller's order).  numpy
+        arrays, or float64 device tensors (anything with ``data_ptr`` and
+        ``is_cuda``) used in place."""
+        n = self.domain.n
+        if getattr(density, "is_cuda", False) or getattr(momentum, "is_cuda", False):
+            import torch
+            if not (density.is_cuda and momentum.is_cuda):
+                raise ValueError("density and momentum must both be device tensors")
+            if density.dtype != torch.float64 or momentum.dtype != torch.float64:
+                raise ValueError("device fields must be float64")
+            if tuple(density.shape) != (n,) or tuple(momentum.shape) != (n, 3):
+                raise ValueError("fields must have shapes (n,) and (n, 3)")
+            if not (density.is_contiguous() and momentum.is_contiguous()):
+                raise ValueError("device fields must be contiguous")
+            torch.cuda.current_stream().synchronize()
+            N.check(N.lib().hlb_set_equilibrium_fields(self._h, density.data_ptr(),
+                                                       momentum.data_ptr(), 1))
+            return
+        rho = np.ascontiguousarray(density, np.float64)
+        m = np.ascontiguousarray(momentum, np.float64)
+        if rho.shape != (n,) or m.shape != (n, 3):
+            raise ValueError("fields must have shapes (n,) and (n, 3)")
+        N.check(N.lib().hlb_set_equilibrium_fields(self._h, rho.ctypes.data_as(C.c_void_p),
+                                                   m.ctypes.data_as(C.c_void_p), 0))
+
     def macroscopic(self):
         rho = np.empty(self.domain.n, np.float64)
         u = np.empty((self.domain.n, 3), np.float64)

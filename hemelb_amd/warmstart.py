@@ -4,9 +4,10 @@ plus iolet-parameter channels to the periodic-state velocity and pressure
 of a vessel; the LB run then starts from the equilibrium of that field.
 
 Input channels (per voxel of a cubic crop): signed distance to the wall in
-voxels (positive in the fluid) divided by 16, inlet pressure amplitude
-(mmHg), period / 1000 steps, tau, and cos / sin of the phase of the cycle.
-Output channels: the velocity in units of 0.01 and (density - 1) / 0.01.
+voxels (positive in the fluid) divided by 16, 10 (tau - 1/2), period / 1000
+steps, half the Stokes-layer thickness, and cos / sin of the phase of the
+cycle.  Output channels: the velocity in units of half the inlet density
+amplitude and (density - 1) in units of that amplitude.
 """
 from __future__ import annotations
 
@@ -21,8 +22,6 @@ from . import _native as N
 CHANNELS = (16, 32, 64, 128)
 IN_CHANNELS = 6
 OUT_CHANNELS = 4
-U_SCALE = 0.01
-RHO_SCALE = 0.01
 
 _SIGS = {
     "hlbn_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -202,29 +201,49 @@ class UNet:
 # ---------------------------------------------------------------------------
 # Training data: periodic states of the plain binary32 solver
 # ---------------------------------------------------------------------------
+# Parameter ranges of the synthetic vessels.  "slow": low viscosity, inlet
+# amplitude given as a lattice density; the plain run from rest needs tens
+# of cycles to become periodic.  "fast": tau 0.6-1.0 with 0.5-2 mmHg
+# amplitudes, where it needs only a few.
+REGIMES = {
+    "slow": dict(tau=(0.52, 0.56), amp=(2e-4, 2e-3), units="lattice"),
+    "fast": dict(tau=(0.6, 1.0), amp=(0.5, 2.0), units="mmHg"),
+}
+
+
 @dataclass
 class RunSpec:
     """One vessel of the training or evaluation set."""
     kind: str            # "cylinder" (stenosed) or "bifurcation" (3 generations)
     seed: int
     tau: float
-    amplitude_mmHg: float
+    amplitude: float     # inlet amplitude in `units`
+    units: str           # "lattice" (density) or "mmHg"
     geometry: object = None
     origin: tuple = None   # crop origin in the geometry's coordinates
     params: tuple = None   # (R, length, severity, width, z0)
 
+    @property
+    def rho_amp(self) -> float:
+        """Inlet density amplitude in lattice units."""
+        if self.units == "lattice":
+            return self.amplitude
+        from .workloads import lattice_density_amplitude
+        return lattice_density_amplitude(self.tau, self.amplitude)
 
-def run_spec(seed: int, side: int = 128) -> RunSpec:
+
+def run_spec(seed: int, side: int = 128, regime: str = "slow") -> RunSpec:
     """Seeded vessel fitting a side^3 crop: even seeds stenosed cylinders
     (radius ~U(6, 14), length side - 8, area reduction ~U(0.2, 0.7), width
-    ~U(1, 3) x radius), odd seeds 3-generation bifurcations (root radius
-    ~U(4.5, 6) scaled with the crop); tau ~U(0.6, 1.0), inlet amplitude
-    ~U(0.5, 2) mmHg."""
+    ~U(1, 3) x radius; sizes for side 128, scaled with the crop), odd seeds
+    3-generation bifurcations (root radius ~U(4.5, 6) x side / 128); tau and
+    inlet amplitude uniform over the regime's ranges."""
     from .geometry import Geometry
 
+    reg = REGIMES[regime]
     rng = np.random.default_rng(10007 * seed + 17)
-    tau = float(rng.uniform(0.6, 1.0))
-    amp = float(rng.uniform(0.5, 2.0))
+    tau = float(rng.uniform(*reg["tau"]))
+    amp = float(rng.uniform(*reg["amp"]))
     scale = side / 128.0
     if seed % 2 == 0:
         R = float(rng.uniform(6.0, 14.0) * scale)
@@ -247,41 +266,72 @@ def run_spec(seed: int, side: int = 128) -> RunSpec:
     c = g.links()[0]
     lo, hi = c.min(0), c.max(0)
     origin = tuple(int(v) for v in lo - (side - (hi - lo + 1)) // 2)
-    return RunSpec(kind, seed, tau, amp, g, origin, params)
+    return RunSpec(kind, seed, tau, amp, reg["units"], g, origin, params)
+
+
+def input_scalars(spec: RunSpec, period: float) -> tuple:
+    """Per-run input channels 1-3: 10 (tau - 1/2), period / 1000 steps and
+    half the Stokes-layer thickness sqrt(nu T / pi) in voxels."""
+    nu = (spec.tau - 0.5) / 3.0
+    return (10.0 * (spec.tau - 0.5), period / 1000.0, 0.5 * math.sqrt(nu * period / math.pi))
+
+
+def output_scales(spec: RunSpec) -> tuple:
+    """(velocity, density) units of the network outputs: the fields are
+    linear in the inlet amplitude at these Reynolds numbers, so the outputs
+    are u / (rho_amp / 2) and (rho - 1) / rho_amp."""
+    a = spec.rho_amp
+    return 0.5 * a, a
 
 
 def input_field(spec: RunSpec, side: int, period: float, phase: float,
                 sdf: np.ndarray | None = None) -> np.ndarray:
-    """[side, side, side, 6] network input for `spec` at `phase` (radians)."""
+    """[side, side, side, 6] network input for `spec` at `phase` (radians):
+    signed distance / 16 (clipped to +-1), input_scalars, cos and sin of
+    the phase."""
     if sdf is None:
         sdf = spec.geometry.signed_distance(spec.origin, (side, side, side))
     x = np.empty((side, side, side, IN_CHANNELS), np.float32)
     x[..., 0] = np.clip(sdf, -16.0, 16.0) / 16.0
-    x[..., 1] = spec.amplitude_mmHg
-    x[..., 2] = period / 1000.0
-    x[..., 3] = spec.tau
+    for k, v in enumerate(input_scalars(spec, period)):
+        x[..., 1 + k] = v
     x[..., 4] = math.cos(phase)
     x[..., 5] = math.sin(phase)
     return x
 
 
+def signed_distances(specs, side: int, workers: int = 16) -> list:
+    """Signed-distance crops of `specs` ([side^3] float32 each), computed in
+    parallel (the native transform releases the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(s):
+        return s.geometry.signed_distance(s.origin, (side, side, side)).reshape(-1)
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        return list(ex.map(one, specs))
+
+
 def _crop_index(spec: RunSpec, coords: np.ndarray, offset, side: int) -> np.ndarray:
     """Flat crop index of each ensemble site of the instance."""
-    local = coords - np.asarray(offset) - np.asarray(spec.origin) + 0
+    local = coords - np.asarray(offset) - np.asarray(spec.origin)
     return (local[:, 0] * side + local[:, 1]) * side + local[:, 2]
 
 
 def _ensemble_of(specs, period):
     from . import workloads as WL
-    parts = [(s.geometry, *s.params, s.tau, s.amplitude_mmHg) for s in specs]
-    return WL.make_ensemble(parts, period, name="warm_start_batch")
+    units = {s.units for s in specs}
+    if len(units) != 1:
+        raise ValueError("an ensemble's runs must share amplitude units")
+    parts = [(s.geometry, *s.params, s.tau, s.amplitude) for s in specs]
+    return WL.make_ensemble(parts, period, name="warm_start_batch",
+                            amplitude_units=units.pop())
 
 
 def run_to_periodic(lbm, ens, period: int, tol: float = 1e-4, max_cycles: int = 80,
                     start_cycle_check: int = 1):
     """Advances the ensemble cycle by cycle; returns per-instance cycles until
-    max_sites |u(t) - u(t - T)| / max |u| < tol (0 where never reached) and
-    the number of cycles run."""
+    max_sites |u(t) - u(t - T)| / max |u| < tol (0 where never reached, -1
+    where the instance went unstable) and the number of cycles run."""
     n = len(ens.instances)
     done = np.zeros(n, np.int64)
     lbm.snapshot_velocity()
@@ -289,12 +339,15 @@ def run_to_periodic(lbm, ens, period: int, tol: float = 1e-4, max_cycles: int = 
     for c in range(1, max_cycles + 1):
         lbm.step(period)
         cycles = c
-        umax = np.array([max(m["velocity_max"], 1e-12) for m in lbm.monitor_instances()])
-        change = lbm.velocity_change_instances(umax)
+        mon = lbm.monitor_instances()
+        umax = np.array([m["velocity_max"] for m in mon])
+        bad = ~np.array([bool(m["stable"]) for m in mon]) | ~np.isfinite(umax)
+        change = lbm.velocity_change_instances(np.where(bad | (umax <= 0), 1.0, umax))
         lbm.snapshot_velocity()
-        newly = (change < tol) & (done == 0) & (c >= start_cycle_check)
+        done[bad & (done == 0)] = -1
+        newly = np.isfinite(change) & (change < tol) & (done == 0) & (c >= start_cycle_check)
         done[newly] = c
-        if (done > 0).all():
+        if (done != 0).all():
             break
     return done, cycles
 
@@ -303,8 +356,8 @@ def simulate(specs, side: int, period: int, phases: int = 8, tol: float = 1e-4,
              max_cycles: int = 80, storage: str = "fp32"):
     """Runs the vessels of `specs` together from rest to their periodic state
     and samples `phases` fields over the next cycle.  Returns (targets
-    [len, phases, side^3, 4] float16, masks [len, side^3] uint8, cycles,
-    seconds)."""
+    [len, phases, side^3, 4] float16 in the output_scales units, masks
+    [len, side^3] uint8, cycles, seconds)."""
     import time as _time
     ens = _ensemble_of(specs, period)
     lbm = ens.solver(storage=storage)
@@ -317,6 +370,7 @@ def simulate(specs, side: int, period: int, phases: int = 8, tol: float = 1e-4,
            for s, i in zip(specs, ens.instances)]
     for k, ix in enumerate(idx):
         masks[k, ix] = 1
+    sc = [output_scales(s) for s in specs]
     start = lbm.GetTimeStep()
     for ph in range(phases):
         target_t = start + int(round(ph * period / phases))
@@ -324,8 +378,8 @@ def simulate(specs, side: int, period: int, phases: int = 8, tol: float = 1e-4,
         rho, u = lbm.macroscopic()
         for k, (inst, ix) in enumerate(zip(ens.instances, idx)):
             sl = inst.sites
-            targets[k, ph, ix, :3] = u[sl] / U_SCALE
-            targets[k, ph, ix, 3] = (rho[sl] - 1.0) / RHO_SCALE
+            targets[k, ph, ix, :3] = u[sl] / sc[k][0]
+            targets[k, ph, ix, 3] = (rho[sl] - 1.0) / sc[k][1]
     lbm.synchronize()
     return targets, masks, cycles, _time.perf_counter() - t0
 
@@ -351,20 +405,30 @@ def equilibrium(rho: np.ndarray, u: np.ndarray, Q: int = 19) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # Training and the warm-start evaluation
 # ---------------------------------------------------------------------------
+def _torch_cuda():
+    """torch with its HIP runtime initialised.  torch ships its own HIP
+    runtime next to the library's (ROCm's); in one process torch's has to
+    initialise first, so every entry point here that touches torch calls
+    this before the library creates a solver or a network."""
+    import torch
+    torch.cuda.init()
+    return torch
+
+
 def _device_sync():
     import torch
     torch.cuda.current_stream().synchronize()
 
 
 def build_dataset(seeds, side: int, periods, batch_runs: int = 64, phases: int = 8,
-                  tol: float = 1e-4, max_cycles: int = 80, log=print):
+                  tol: float = 1e-4, max_cycles: int = 80, regime: str = "slow", log=print):
     """Periodic-state fields of the runs `seeds`, simulated batch_runs at a
     time (each batch with its own period), kept in device memory:
     dict with sdf [R, S^3] f16, targets [R, phases, S^3, 4] f16, masks
-    [R, S^3] u8, counts, and per run (spec, period, start step, cycles)."""
+    [R, S^3] u8, counts, and per run (spec, period, cycles)."""
     import time as _time
 
-    import torch
+    torch = _torch_cuda()
     dev = torch.device("cuda")
     R = len(seeds)
     n3 = side ** 3
@@ -375,32 +439,38 @@ def build_dataset(seeds, side: int, periods, batch_runs: int = 64, phases: int =
     for b0 in range(0, R, batch_runs):
         bs = seeds[b0:b0 + batch_runs]
         period = int(periods[(b0 // batch_runs) % len(periods)])
-        specs = [run_spec(s, side) for s in bs]
+        specs = [run_spec(s, side, regime) for s in bs]
         t0 = _time.perf_counter()
         targets, masks, cycles, secs = simulate(specs, side, period, phases, tol, max_cycles)
-        for k, s in enumerate(specs):
-            d = s.geometry.signed_distance(s.origin, (side, side, side)).reshape(-1)
+        for k, (s, d) in enumerate(zip(specs, signed_distances(specs, side))):
             sdf[b0 + k] = torch.from_numpy(np.clip(d, -16, 16) / 16.0).to(dev, torch.float16)
-            runs.append(dict(spec=s, period=period, cycles=int(cycles[k])))
+            # only runs that reached a periodic state with finite fields train
+            ok = bool(cycles[k] > 0 and np.isfinite(targets[k]).all())
+            if not ok:
+                masks[k] = 0
+            runs.append(dict(spec=s, period=period, cycles=int(cycles[k]), ok=ok))
         tg[b0:b0 + len(bs)] = torch.from_numpy(targets).to(dev)
         mk[b0:b0 + len(bs)] = torch.from_numpy(masks).to(dev)
+        conv = cycles[cycles > 0]
         log(f"[warmstart] simulated runs {b0}..{b0 + len(bs) - 1} (period {period}): "
             f"{secs:.1f}s LB, {_time.perf_counter() - t0:.1f}s total, "
-            f"periodic after {np.median(cycles):.0f} cycles (median), "
-            f"{(cycles == 0).sum()} not converged")
+            f"periodic after {np.median(conv) if conv.size else 0:.0f} cycles (median, max "
+            f"{conv.max() if conv.size else 0}), {(cycles == 0).sum()} not converged, "
+            f"{(cycles < 0).sum()} unstable")
     return dict(sdf=sdf, targets=tg, masks=mk, counts=mk.sum(1).double().cpu().numpy(),
                 runs=runs, phases=phases, side=side)
 
 
-def _input_tensor(data, r: int, phase: float):
-    import torch
-    run = data["runs"][r]
-    s = data["side"]
-    x = torch.empty((s ** 3, IN_CHANNELS), dtype=torch.float32, device="cuda")
-    x[:, 0] = data["sdf"][r].float()
-    x[:, 1] = run["spec"].amplitude_mmHg
-    x[:, 2] = run["period"] / 1000.0
-    x[:, 3] = run["spec"].tau
+def _input_tensor(sdf, spec: RunSpec, period: float, phase: float, out=None):
+    """Device input [S^3, 6] from a device signed-distance crop (already
+    clipped and divided by 16)."""
+    torch = _torch_cuda()
+    n = sdf.numel()
+    x = out if out is not None else torch.empty((n, IN_CHANNELS), dtype=torch.float32,
+                                                device=sdf.device)
+    x[:, 0] = sdf.reshape(-1).float()
+    for k, v in enumerate(input_scalars(spec, period)):
+        x[:, 1 + k] = v
     x[:, 4] = math.cos(phase)
     x[:, 5] = math.sin(phase)
     return x
@@ -414,35 +484,44 @@ def _sample_phase(ph: int, phases: int) -> float:
 def train(net: "UNet", data, epochs: int, lr: float = 1e-3, seed: int = 0,
           channel_weights=(1.0, 1.0, 1.0, 1.0), log=print, time_budget_s: float = 1e30):
     """Adam over every (run, phase) sample, one per step, cosine-decayed
-    learning rate.  Returns the per-epoch mean losses."""
+    learning rate (over the epochs or the time budget, whichever ends
+    first).  Returns the per-epoch mean losses."""
     import time as _time
 
-    import torch
+    torch = _torch_cuda()
     rng = np.random.default_rng(seed)
-    R, phases = len(data["runs"]), data["phases"]
+    good = np.array([r for r, run in enumerate(data["runs"]) if run["ok"]], np.int64)
+    R, phases = len(good), data["phases"]
     total = epochs * R * phases
     done, losses, t0 = 0, [], _time.perf_counter()
     w = np.asarray(channel_weights, np.float32)
+    x = torch.empty((data["side"] ** 3, IN_CHANNELS), dtype=torch.float32, device="cuda")
     for ep in range(epochs):
         order = rng.permutation(R * phases)
-        acc = 0.0
+        acc, cnt = 0.0, 0
         for i in order:
             r, ph = divmod(int(i), phases)
-            x = _input_tensor(data, r, _sample_phase(ph, phases))
+            r = int(good[r])
+            run = data["runs"][r]
+            _input_tensor(data["sdf"][r], run["spec"], run["period"],
+                          _sample_phase(ph, phases), out=x)
             t = data["targets"][r, ph].float()
             _device_sync()
             net.forward(x, device=True)
             loss = net.backward(t, data["masks"][r], w, float(data["counts"][r]), device=True)
+            if not math.isfinite(loss):
+                raise RuntimeError(f"non-finite training loss at step {done} (run {r})")
             elapsed = _time.perf_counter() - t0
             frac = min(1.0, max(done / max(total, 1), elapsed / time_budget_s))
             net.adam(lr=lr * 0.5 * (1.0 + math.cos(math.pi * frac)))
             acc += loss
+            cnt += 1
             done += 1
             if done % 500 == 0:
                 log(f"[warmstart] step {done}: loss {loss:.5f} ({elapsed:.0f}s)")
             if elapsed > time_budget_s:
                 break
-        losses.append(acc / max(done - ep * R * phases, 1))
+        losses.append(acc / max(cnt, 1))
         log(f"[warmstart] epoch {ep}: mean loss {losses[-1]:.5f} "
             f"({_time.perf_counter() - t0:.0f}s)")
         if _time.perf_counter() - t0 > time_budget_s:
@@ -450,44 +529,73 @@ def train(net: "UNet", data, epochs: int, lr: float = 1e-3, seed: int = 0,
     return losses
 
 
-def predict_sites(net: "UNet", spec: RunSpec, side: int, period: int, phase: float,
-                  coords: np.ndarray, offset) -> tuple:
-    """Predicted (rho, u) at the ensemble sites `coords` of the instance."""
-    x = input_field(spec, side, period, phase)[None]
-    y = net.forward(x)[0].reshape(-1, OUT_CHANNELS)
-    ix = _crop_index(spec, coords, offset, side)
-    u = y[ix, :3].astype(np.float64) * U_SCALE
-    rho = 1.0 + y[ix, 3].astype(np.float64) * RHO_SCALE
-    return rho, u
+def predict_fields(net: "UNet", specs, ens, side: int, period: int, phase: float,
+                   sdfs=None):
+    """Predicted density and momentum of every ensemble site as float64
+    device tensors ([n], [n, 3]), the network run once per instance on the
+    device."""
+    torch = _torch_cuda()
+    dev = torch.device("cuda")
+    n = ens.domain.n
+    rho = torch.ones(n, dtype=torch.float64, device=dev)
+    mom = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    if sdfs is None:
+        sdfs = signed_distances(specs, side)
+    x = torch.empty((side ** 3, IN_CHANNELS), dtype=torch.float32, device=dev)
+    y = torch.empty((side ** 3, OUT_CHANNELS), dtype=torch.float32, device=dev)
+    for s, inst, d in zip(specs, ens.instances, sdfs):
+        sd = torch.from_numpy(np.clip(d, -16, 16) / 16.0).to(dev, torch.float32)
+        _input_tensor(sd, s, period, phase, out=x)
+        _device_sync()
+        net.forward(x, y, device=True)
+        ix = torch.from_numpy(_crop_index(s, ens.domain.coords[inst.sites], inst.offset,
+                                          side)).to(dev)
+        su, sr = output_scales(s)
+        yi = y.index_select(0, ix).double()
+        r = 1.0 + yi[:, 3] * sr
+        rho[inst.sites] = r
+        mom[inst.sites] = yi[:, :3] * su * r[:, None]
+    return rho, mom
 
 
 def evaluate(net: "UNet", seeds, side: int, period: int, tol: float = 1e-4,
-             max_cycles: int = 80, log=print) -> dict:
+             max_cycles: int = 80, regime: str = "slow", log=print) -> dict:
     """Held-out vessels run to the periodic state from rest (the plain
     method) and from the network's prediction; returns the cycles and wall
     times of both and the difference of the converged fields."""
     import time as _time
-    specs = [run_spec(s, side) for s in seeds]
+    _torch_cuda()
+    specs = [run_spec(s, side, regime) for s in seeds]
     ens = _ensemble_of(specs, period)
     # plain run from rest
     cold = ens.solver(storage="fp32")
+    cold.synchronize()
     t0 = _time.perf_counter()
     c_cold, n_cold = run_to_periodic(cold, ens, period, tol, max_cycles)
     wall_cold = _time.perf_counter() - t0
+    _, u_ref0 = cold.macroscopic()   # periodic field at the phase of step 1
     # warm start: equilibrium of the predicted field at the phase of step 1
     warm = ens.solver(storage="fp32")
+    warm.synchronize()
     t0 = _time.perf_counter()
-    f = np.empty((ens.domain.n, 19))
-    phase0 = 0.0
-    for s, inst in zip(specs, ens.instances):
-        rho, u = predict_sites(net, s, side, period, phase0, ens.domain.coords[inst.sites],
-                               inst.offset)
-        f[inst.sites] = equilibrium(rho, u)
-    warm.set_distributions(f)
+    sdfs = signed_distances(specs, side)
+    t_sdf = _time.perf_counter() - t0
+    rho, mom = predict_fields(net, specs, ens, side, period, 0.0, sdfs)
+    warm.set_equilibrium_fields(rho, mom)
     warm.SetTimeStep(1)
     t_pred = _time.perf_counter() - t0
     c_warm, n_warm = run_to_periodic(warm, ens, period, tol, max_cycles)
     wall_warm = _time.perf_counter() - t0
+    u_pred = (mom / rho[:, None]).cpu().numpy()
+    pred_l2, pred_max = [], []
+    for inst in ens.instances:
+        a, b = u_pred[inst.sites], u_ref0[inst.sites]
+        pred_l2.append(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)))
+        pred_max.append(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)))
+    # instances the plain method cannot run stably are reported, not scored
+    unstable = (c_cold < 0) | (c_warm < 0)
+    if unstable.all():
+        raise RuntimeError("every instance went unstable")
     # converged fields at the same phase
     steps = max(n_cold, n_warm) * period
     cold.step(steps + 1 - cold.GetTimeStep())
@@ -495,23 +603,31 @@ def evaluate(net: "UNet", seeds, side: int, period: int, tol: float = 1e-4,
     _, u_c = cold.macroscopic()
     _, u_w = warm.macroscopic()
     diffs = []
-    for inst in ens.instances:
+    for inst, bad in zip(ens.instances, unstable):
+        if bad:
+            continue
         a, b = u_w[inst.sites], u_c[inst.sites]
         diffs.append(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)))
     conv = (c_cold > 0) & (c_warm > 0)
     ratio = (c_cold[conv] / c_warm[conv]) if conv.any() else np.array([np.nan])
-    res = dict(instances=len(specs), period=period, tol=tol,
+    res = dict(regime=regime, instances=len(specs), period=period, tol=tol,
+               unstable_cold=np.flatnonzero(c_cold < 0).tolist(),
+               unstable_warm=np.flatnonzero(c_warm < 0).tolist(),
                cycles_cold=c_cold.tolist(), cycles_warm=c_warm.tolist(),
+               cycles_run_cold=int(n_cold), cycles_run_warm=int(n_warm),
                cycle_ratio_median=float(np.median(ratio)),
                cycle_ratio_total=float(c_cold[conv].sum() / max(c_warm[conv].sum(), 1)),
-               wall_cold_s=wall_cold, wall_warm_s=wall_warm, prediction_s=t_pred,
-               wall_ratio=wall_cold / max(wall_warm, 1e-9),
+               wall_cold_s=wall_cold, wall_warm_s=wall_warm, sdf_s=t_sdf,
+               prediction_s=t_pred, wall_ratio=wall_cold / max(wall_warm, 1e-9),
+               prediction_rel_l2_median=float(np.median(pred_l2)),
+               prediction_rel_max_median=float(np.median(pred_max)),
                converged_field_max_rel_diff=float(np.max(diffs)),
                converged_field_median_rel_diff=float(np.median(diffs)))
-    log(f"[warmstart] eval: cycles cold {np.sum(c_cold)} warm {np.sum(c_warm)} "
-        f"(median ratio {res['cycle_ratio_median']:.2f}); wall {wall_cold:.1f}s vs "
-        f"{wall_warm:.1f}s (x{res['wall_ratio']:.2f}); field diff max "
-        f"{res['converged_field_max_rel_diff']:.2e}")
+    log(f"[warmstart] eval: cycles until all periodic cold {n_cold} warm {n_warm}; "
+        f"per-instance median ratio {res['cycle_ratio_median']:.2f}; wall {wall_cold:.2f}s "
+        f"vs {wall_warm:.2f}s (x{res['wall_ratio']:.2f}, prediction {t_pred:.2f}s, "
+        f"median relative L2 error {res['prediction_rel_l2_median']:.3f}); "
+        f"field diff max {res['converged_field_max_rel_diff']:.2e}")
     return res
 
 
@@ -521,41 +637,53 @@ def main(argv=None) -> int:
     import sys
     import time as _time
     ap = argparse.ArgumentParser(description="train and evaluate the learned warm start")
-    ap.add_argument("--side", type=int, default=128)
+    ap.add_argument("--side", type=int, default=64)
+    ap.add_argument("--regime", choices=sorted(REGIMES), default="slow")
     ap.add_argument("--train-runs", type=int, default=512)
     ap.add_argument("--eval-runs", type=int, default=64)
     ap.add_argument("--batch-runs", type=int, default=64)
-    ap.add_argument("--periods", default="1000,700,1400")
-    ap.add_argument("--eval-period", type=int, default=1000)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--periods", default="500,400,700")
+    ap.add_argument("--eval-period", type=int, default=500)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--max-cycles", type=int, default=60)
+    ap.add_argument("--max-cycles", type=int, default=200)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--train-seconds", type=float, default=1e30)
     ap.add_argument("--out", default="")
     ap.add_argument("--save-weights", default="")
+    ap.add_argument("--load-weights", default="",
+                    help="evaluate saved parameters (no data generation or training)")
     a = ap.parse_args(argv)
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
-    t0 = _time.perf_counter()
-    periods = [int(v) for v in a.periods.split(",")]
-    data = build_dataset(list(range(a.train_runs)), a.side, periods, a.batch_runs, 8, a.tol,
-                         a.max_cycles, log)
-    t_data = _time.perf_counter() - t0
+    _torch_cuda()
     net = UNet(a.side, 1)
-    net.set_params(init_params())
-    t1 = _time.perf_counter()
-    losses = train(net, data, a.epochs, a.lr, log=log, time_budget_s=a.train_seconds)
-    t_train = _time.perf_counter() - t1
-    f_ms, b_ms = net.times()
-    if a.save_weights:
-        np.save(a.save_weights, net.params())
-    del data
+    info = dict(side=a.side)
+    if a.load_weights:
+        net.set_params(np.load(a.load_weights))
+        info.update(weights=a.load_weights)
+    else:
+        t0 = _time.perf_counter()
+        periods = [int(v) for v in a.periods.split(",")]
+        data = build_dataset(list(range(a.train_runs)), a.side, periods, a.batch_runs, 8,
+                             a.tol, a.max_cycles, a.regime, log)
+        t_data = _time.perf_counter() - t0
+        net.set_params(init_params())
+        t1 = _time.perf_counter()
+        losses = train(net, data, a.epochs, a.lr, log=log, time_budget_s=a.train_seconds)
+        t_train = _time.perf_counter() - t1
+        f_ms, b_ms = net.times()
+        if a.save_weights:
+            np.save(a.save_weights, net.params())
+        train_cycles = [r["cycles"] for r in data["runs"] if r["ok"]]
+        del data
+        info.update(train_runs=a.train_runs, epochs_run=len(losses), losses=losses,
+                    data_seconds=t_data, train_seconds=t_train, forward_ms=f_ms,
+                    backward_ms=b_ms, train_cycles_median=float(np.median(train_cycles)))
     res = evaluate(net, list(range(100000, 100000 + a.eval_runs)), a.side, a.eval_period,
-                   a.tol, a.max_cycles, log)
-    res.update(side=a.side, train_runs=a.train_runs, epochs_run=len(losses), losses=losses,
-               data_seconds=t_data, train_seconds=t_train, forward_ms=f_ms, backward_ms=b_ms)
+                   a.tol, a.max_cycles, a.regime, log)
+    res.update(info)
     line = json.dumps(res)
     print(line, flush=True)
     if a.out:

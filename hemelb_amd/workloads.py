@@ -117,7 +117,7 @@ class Instance:
     width: float               # its width (voxels)
     z0: float                  # its axial position (voxels)
     tau: float
-    amplitude_mmHg: float      # inlet pressure amplitude
+    amplitude_mmHg: float      # inlet amplitude (mmHg; lattice density if built so)
     offset: tuple              # placement of the instance in the ensemble
     sites: slice               # its rows of the ensemble domain
     inlets: slice = slice(0, 0)   # its entries of the ensemble's inlet list
@@ -162,14 +162,26 @@ class Ensemble:
                       [], [])
 
 
+def lattice_density_amplitude(tau: float, amplitude_mmHg: float,
+                              voxel_size_m: float = 5e-6) -> float:
+    """Lattice density amplitude of an inlet pressure amplitude for blood at
+    relaxation time tau on voxels of voxel_size_m (time step from the lattice
+    viscosity, UnitConverter.cc)."""
+    dt = (tau - 0.5) * L.Cs2 * voxel_size_m ** 2 * BLOOD_DENSITY / BLOOD_VISCOSITY
+    units = L.UnitConverter(dt, voxel_size_m, BLOOD_DENSITY, 80.0)
+    return units.ConvertPressureDifferenceToLatticeUnits(amplitude_mmHg) / L.Cs2
+
+
 def make_ensemble(parts: list, period: float, voxel_size_m: float = 5e-6,
-                  steps: int = 4000, name: str = "ensemble") -> Ensemble:
+                  steps: int = 4000, name: str = "ensemble",
+                  amplitude_units: str = "mmHg") -> Ensemble:
     """An ensemble of independent vessels: parts = [(geometry, Instance
-    fields radius, length, severity, width, z0, tau, amplitude_mmHg)] with one
+    fields radius, length, severity, width, z0, tau, amplitude)] with one
     inlet and any number of outlets each.  Inlet pressure 80 + A cos(2 pi t /
     period) mmHg, outlets 80 mmHg (the reference pressure, density 1), D3Q19
-    LBGK, BFL walls, Nash iolets.  Each vessel sits in its own aligned cube of
-    8^3 blocks, so that the Morton order keeps it contiguous."""
+    LBGK, BFL walls, Nash iolets; amplitude_units "lattice" gives A as a
+    lattice density amplitude instead.  Each vessel sits in its own aligned
+    cube of 8^3 blocks, so that the Morton order keeps it contiguous."""
     import numpy as np
 
     doms = [g.project(19) for g, *_ in parts]
@@ -190,9 +202,12 @@ def make_ensemble(parts: list, period: float, voxel_size_m: float = 5e-6,
         ids[d.site_type == 3] += n_out
         pieces.append(Domain(19, d.coords + off, d.site_type, d.wall_mask, d.iolet_mask, ids,
                              d.cut, d.normal, [], []))
-        dt = (tau - 0.5) * L.Cs2 * voxel_size_m ** 2 * BLOOD_DENSITY / BLOOD_VISCOSITY
-        units = L.UnitConverter(dt, voxel_size_m, BLOOD_DENSITY, 80.0)
-        rho_amp = units.ConvertPressureDifferenceToLatticeUnits(amp) / L.Cs2
+        if amplitude_units == "lattice":
+            rho_amp = float(amp)
+        elif amplitude_units == "mmHg":
+            rho_amp = lattice_density_amplitude(tau, amp, voxel_size_m)
+        else:
+            raise ValueError(f"unknown amplitude units {amplitude_units!r}")
         for p in d.inlets:
             ins.append(L.InOutLetCosine(tuple(np.asarray(p.position) + off), tuple(p.normal),
                                         density_mean=1.0, density_amp=rho_amp, period=period))

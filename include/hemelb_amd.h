@@ -168,6 +168,14 @@ int hlb_step(hlb_solver* s, int64_t n);
  * Code/extraction/LocalDistributionInput.cc:38-80). */
 int hlb_get_distributions(hlb_solver* s, double* f_out);
 int hlb_set_distributions(hlb_solver* s, const double* f_in);
+/* fOld = feq(density[i], momentum[i]) at every site i, in the caller's order
+ * (density [n], momentum [n][3]), with the arithmetic of
+ * Lattice::CalculateFeq (Lattice.h:236-320): the equilibrium initial
+ * condition of InitialCondition.hpp:39-52 generalised from one uniform
+ * value to a field (e.g. a predicted flow field).  `device` != 0: the
+ * arrays are device pointers.  The time is left unchanged. */
+int hlb_set_equilibrium_fields(hlb_solver* s, const double* density, const double* momentum,
+                               int32_t device);
 
 /* Density [n] and velocity [n][3] of fOld in the caller's order (either may
  * be NULL), as MacroscopicPropertyCache / LbDataSourceIterator compute them. */

@@ -211,3 +211,61 @@ def test_stress_outputs_match_oracle(label, make):
     got = gpu.stress()
     for k in want:
         np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+
+
+def _oracle_feq(Q, rho, m):
+    from oracle import bindings as O
+    P = lambda a: a.ctypes.data_as(O.C.c_void_p)  # noqa: E731
+    out = np.empty((len(rho), Q))
+    row = np.empty(Q)
+    for i in range(len(rho)):
+        O.lib().orc_feq(Q, float(rho[i]), P(np.ascontiguousarray(m[i])), P(row))
+        out[i] = row
+    return out
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_equilibrium_fields_match_oracle(layout):
+    """hlb_set_equilibrium_fields: per-site feq bit-exact with the oracle's
+    Lattice::CalculateFeq, from host arrays and from device tensors, and the
+    run continues in step with the oracle started from the same fOld."""
+    from hemelb_amd import _native as N
+    from tests.devmem import DeviceArray
+    case = K.cylinder_case(6.0, 24, 19, "MRT", "BFL", "womersley", tau=0.7)
+    n = case.domain.n
+    rng = np.random.default_rng(7)
+    rho = rng.uniform(0.97, 1.03, n)
+    m = rng.normal(0.0, 0.01, (n, 3)) * rho[:, None]
+    want = _oracle_feq(19, rho, m)
+    print(f"[parity] equilibrium fields layout={layout} n={n}", flush=True)
+    gpu = case.product(layout=layout)
+    gpu.SetTimeStep(4)
+    gpu.set_equilibrium_fields(rho, m)
+    assert gpu.GetTimeStep() == 4
+    np.testing.assert_array_equal(gpu.get_distributions(), want)
+    dev = case.product(layout=layout)
+    d_rho, d_m = DeviceArray(rho), DeviceArray(m)
+    N.check(N.lib().hlb_set_equilibrium_fields(dev._h, d_rho.ptr, d_m.ptr, 1))
+    np.testing.assert_array_equal(dev.get_distributions(), want)
+    with pytest.raises(ValueError):
+        dev.set_equilibrium_fields(rho[:-1], m)
+    orc = case.oracle()
+    orc.set_f(want)
+    orc.step(9)
+    dev.step(9)
+    np.testing.assert_array_equal(dev.get_distributions(), orc.get_f())
+
+
+def test_equilibrium_fields_reduced_precision():
+    case = K.cylinder_case(6.0, 24, 19, "LBGK", "BFL", "womersley", tau=0.7)
+    n = case.domain.n
+    rng = np.random.default_rng(8)
+    rho = rng.uniform(0.97, 1.03, n)
+    m = rng.normal(0.0, 0.01, (n, 3)) * rho[:, None]
+    want = _oracle_feq(19, rho, m)
+    for storage, tol in (("fp32", 2e-7), ("fp16", 2e-3)):
+        g = case.product(storage=storage)
+        g.set_equilibrium_fields(rho, m)
+        err = np.abs(g.get_distributions() - want).max() / np.abs(want).max()
+        print(f"[parity] equilibrium fields {storage}: max rel error {err:.2e}", flush=True)
+        assert err < tol

@@ -21,14 +21,30 @@ def test_equilibrium_has_the_requested_moments():
 
 def test_run_specs_are_seeded_and_fit_the_crop():
     side = 64
-    for seed in (0, 1, 2, 3):
-        a, b = WS.run_spec(seed, side), WS.run_spec(seed, side)
-        assert (a.kind, a.tau, a.amplitude_mmHg, a.origin) == (b.kind, b.tau, b.amplitude_mmHg,
-                                                               b.origin)
-        assert a.kind == ("cylinder" if seed % 2 == 0 else "bifurcation")
-        assert 0.6 <= a.tau <= 1.0 and 0.5 <= a.amplitude_mmHg <= 2.0
-        c = a.geometry.links()[0] - np.asarray(a.origin)
-        assert c.min() >= 0 and c.max() < side
+    for regime in ("slow", "fast"):
+        lo_t, hi_t = WS.REGIMES[regime]["tau"]
+        lo_a, hi_a = WS.REGIMES[regime]["amp"]
+        for seed in (0, 1, 2, 3):
+            a, b = WS.run_spec(seed, side, regime), WS.run_spec(seed, side, regime)
+            assert (a.kind, a.tau, a.amplitude, a.origin) == (b.kind, b.tau, b.amplitude,
+                                                              b.origin)
+            assert a.kind == ("cylinder" if seed % 2 == 0 else "bifurcation")
+            assert lo_t <= a.tau <= hi_t and lo_a <= a.amplitude <= hi_a
+            c = a.geometry.links()[0] - np.asarray(a.origin)
+            assert c.min() >= 0 and c.max() < side
+
+
+def test_lattice_amplitude_of_a_pressure_amplitude():
+    """1 mmHg at tau 0.8 on 5 um voxels of blood: dt = (tau - 1/2) dx^2 rho /
+    (3 mu), and the density amplitude is dp dt^2 / (rho dx^2 Cs2)."""
+    from hemelb_amd import workloads as WL
+    s = WS.RunSpec("cylinder", 0, 0.8, 1.0, "mmHg")
+    dx, rho, mu = 5e-6, 1000.0, 0.004
+    dt = 0.3 * dx * dx * rho / (3 * mu)
+    want = 133.3223684 * dt * dt / (rho * dx * dx) * 3
+    np.testing.assert_allclose(s.rho_amp, want, rtol=1e-6)
+    np.testing.assert_allclose(WL.lattice_density_amplitude(0.8, 1.0), want, rtol=1e-6)
+    assert WS.RunSpec("cylinder", 0, 0.8, 3e-4, "lattice").rho_amp == 3e-4
 
 
 def test_input_channels_and_crop_index():
@@ -36,9 +52,11 @@ def test_input_channels_and_crop_index():
     s = WS.run_spec(2, side)
     x = WS.input_field(s, side, 700, math.pi / 2)
     assert x.shape == (side, side, side, 6)
-    assert np.allclose(x[..., 1], s.amplitude_mmHg) and np.allclose(x[..., 2], 0.7)
-    assert np.allclose(x[..., 3], s.tau) and np.allclose(x[..., 4], 0, atol=1e-7)
-    assert np.allclose(x[..., 5], 1.0)
+    nu = (s.tau - 0.5) / 3
+    assert np.allclose(x[..., 1], 10 * (s.tau - 0.5)) and np.allclose(x[..., 2], 0.7)
+    assert np.allclose(x[..., 3], 0.5 * math.sqrt(nu * 700 / math.pi))
+    assert np.allclose(x[..., 4], 0, atol=1e-7) and np.allclose(x[..., 5], 1.0)
+    assert WS.output_scales(s) == (0.5 * s.amplitude, s.amplitude)
     coords = s.geometry.links()[0]
     ix = WS._crop_index(s, coords, (0, 0, 0), side)
     sdf = x[..., 0].reshape(-1)
