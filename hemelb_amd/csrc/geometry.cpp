@@ -16,12 +16,14 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -407,7 +409,9 @@ void voxelise(const Shape& shape, const int32_t lo[3], const int32_t hi[3],
     d.radius = io.r;
     (io.type == HLBG_CUT_INLET ? g->inlets : g->outlets).push_back(d);
   }
+  // Fluid sites in block order (block-major, z fastest inside a block).
   const int bs = block_size;
+  std::vector<int32_t> sites;
   for (int bx = 0; bx < occ.nb[0]; ++bx)
     for (int by = 0; by < occ.nb[1]; ++by)
       for (int bz = 0; bz < occ.nb[2]; ++bz) {
@@ -417,75 +421,103 @@ void voxelise(const Shape& shape, const int32_t lo[3], const int32_t hi[3],
             for (int lz = 0; lz < bs; ++lz) {
               int x = bx * bs + lx, y = by * bs + ly, z = bz * bs + lz;
               if (!occ.get(x, y, z)) continue;
-              g->coords.push_back(x);
-              g->coords.push_back(y);
-              g->coords.push_back(z);
-              V3 p{(double)(x - 1 + lo[0]), (double)(y - 1 + lo[1]),
-                   (double)(z - 1 + lo[2])};
-              LinkSet ls;
-              bool any = false, wall = false;
-              V3 nrm_acc{0, 0, 0};
-              for (int k = 0; k < 26; ++k) {
-                ls.type[k] = HLBG_CUT_NONE;
-                ls.iolet[k] = -1;
-                ls.dist[k] = -1.0f;
-                const int* c = kMoore.c[k];
-                if (occ.get(x + c[0], y + c[1], z + c[2])) continue;
-                any = true;
-                V3 cv{(double)c[0], (double)c[1], (double)c[2]};
-                double best = 1e300;
-                int type = HLBG_CUT_WALL, id = -1;
-                V3 nrm{0, 0, 0};
-                double tw;
-                V3 nw;
-                if (shape.wall_exit(p, cv, &tw, &nw) && tw > 0.0 &&
-                    tw <= 1.0 + 1e-9) {
-                  best = tw;
-                  nrm = nw;
-                }
-                for (auto& io : shape.iolets) {
-                  double ti;
-                  if (io.hit(p, cv, &ti) && ti < best) {
-                    best = ti;
-                    type = io.type;
-                    id = io.id;
-                  }
-                }
-                if (best > 1.0) {
-                  // No hit inside the link (round-off at a tangency): as the
-                  // geometry tool does, place the cut at the far end.
-                  best = 1.0;
-                  if (type == HLBG_CUT_WALL && !shape.wall_exit(p, cv, &tw, &nrm))
-                    nrm = V3{0, 0, 0};
-                }
-                ls.type[k] = (uint8_t)type;
-                ls.iolet[k] = id;
-                ls.dist[k] = (float)best;
-                if (type == HLBG_CUT_WALL) {
-                  wall = true;
-                  nrm_acc = nrm_acc + nrm;
-                }
-              }
-              if (any) {
-                g->link_index.push_back((int32_t)g->links.size());
-                g->links.push_back(ls);
-              } else {
-                g->link_index.push_back(-1);
-              }
-              g->has_normal.push_back(wall ? 1 : 0);
-              if (wall) {
-                double nn = norm(nrm_acc);
-                V3 u = nn > 0 ? nrm_acc * (1.0 / nn) : V3{0, 0, 1};
-                g->normal.push_back((float)u.x);
-                g->normal.push_back((float)u.y);
-                g->normal.push_back((float)u.z);
-              } else {
-                g->normal.push_back(0.0f);
-                g->normal.push_back(0.0f);
-                g->normal.push_back(0.0f);
-              }
+              sites.push_back(x);
+              sites.push_back(y);
+              sites.push_back(z);
             }
       }
+  const int64_t n = (int64_t)sites.size() / 3;
+  // Links of every site: independent per site, computed by a pool of host
+  // threads over contiguous chunks (the cut searches dominate set-up time).
+  std::vector<LinkSet> ls_all((size_t)n);
+  std::vector<uint8_t> any_all((size_t)n), wall_all((size_t)n);
+  std::vector<float> nrm_all((size_t)n * 3);
+  auto work = [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      const int32_t x = sites[3 * i], y = sites[3 * i + 1], z = sites[3 * i + 2];
+      V3 p{(double)(x - 1 + lo[0]), (double)(y - 1 + lo[1]), (double)(z - 1 + lo[2])};
+      LinkSet& ls = ls_all[(size_t)i];
+      bool any = false, wall = false;
+      V3 nrm_acc{0, 0, 0};
+      for (int k = 0; k < 26; ++k) {
+        ls.type[k] = HLBG_CUT_NONE;
+        ls.iolet[k] = -1;
+        ls.dist[k] = -1.0f;
+        const int* c = kMoore.c[k];
+        if (occ.get(x + c[0], y + c[1], z + c[2])) continue;
+        any = true;
+        V3 cv{(double)c[0], (double)c[1], (double)c[2]};
+        double best = 1e300;
+        int type = HLBG_CUT_WALL, id = -1;
+        V3 nrm{0, 0, 0};
+        double tw;
+        V3 nw;
+        if (shape.wall_exit(p, cv, &tw, &nw) && tw > 0.0 && tw <= 1.0 + 1e-9) {
+          best = tw;
+          nrm = nw;
+        }
+        for (auto& io : shape.iolets) {
+          double ti;
+          if (io.hit(p, cv, &ti) && ti < best) {
+            best = ti;
+            type = io.type;
+            id = io.id;
+          }
+        }
+        if (best > 1.0) {
+          // No hit inside the link (round-off at a tangency): as the
+          // geometry tool does, place the cut at the far end.
+          best = 1.0;
+          if (type == HLBG_CUT_WALL && !shape.wall_exit(p, cv, &tw, &nrm)) nrm = V3{0, 0, 0};
+        }
+        ls.type[k] = (uint8_t)type;
+        ls.iolet[k] = id;
+        ls.dist[k] = (float)best;
+        if (type == HLBG_CUT_WALL) {
+          wall = true;
+          nrm_acc = nrm_acc + nrm;
+        }
+      }
+      any_all[(size_t)i] = any ? 1 : 0;
+      wall_all[(size_t)i] = wall ? 1 : 0;
+      V3 u{0, 0, 0};
+      if (wall) {
+        double nn = norm(nrm_acc);
+        u = nn > 0 ? nrm_acc * (1.0 / nn) : V3{0, 0, 1};
+      }
+      nrm_all[3 * i] = (float)u.x;
+      nrm_all[3 * i + 1] = (float)u.y;
+      nrm_all[3 * i + 2] = (float)u.z;
+    }
+  };
+  const int64_t chunk = 4096;
+  const int nthreads = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                            (n + chunk - 1) / chunk}));
+  if (nthreads == 1) {
+    work(0, n);
+  } else {
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back([&] {
+        for (int64_t i0; (i0 = next.fetch_add(chunk)) < n;) work(i0, std::min(n, i0 + chunk));
+      });
+    for (auto& th : pool) th.join();
+  }
+  g->coords = std::move(sites);
+  g->link_index.reserve((size_t)n);
+  g->has_normal.reserve((size_t)n);
+  g->normal = std::move(nrm_all);
+  for (int64_t i = 0; i < n; ++i) {
+    if (any_all[(size_t)i]) {
+      g->link_index.push_back((int32_t)g->links.size());
+      g->links.push_back(ls_all[(size_t)i]);
+    } else {
+      g->link_index.push_back(-1);
+    }
+    g->has_normal.push_back(wall_all[(size_t)i]);
+  }
 }
 
 int fail(const std::string& msg) {
