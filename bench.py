@@ -8,9 +8,13 @@ distances, a Ladd/Womersley root inlet with a 0.05 peak centreline velocity
 and 16 Nash zero-gauge pressure outlets.
 
 A step is one full LB timestep over every fluid site, inputs resident in HBM.
-Prints one JSON line: MLUPS, the dominant kernel's HBM roofline, the rates of
-the reduced-precision storage forms on the same workload and the CPU oracle's
-rate on a bounded sample of it.
+Prints one JSON line with:
+* MLUPS and the dominant kernel's HBM roofline, beside the rate of a plain
+  device copy of the same bytes (measured in a child process);
+* the rates of the reduced-precision storage forms on the same workload;
+* the wall time from rest to the periodic state (binary64 and the FP32
+  form);
+* the CPU oracle's rate on a bounded sample of the workload.
 
 ``--ensemble`` instead times the batched form: 96 seeded stenosed cylinders
 (radius ~U(8, 40), D3Q19 LBGK, tau ~U(0.6, 1.0) per vessel, FP32 storage,
@@ -63,6 +67,61 @@ def cpu_baseline(dom, tau, inlets, outlets, steps):
     sim.step(steps)
     dt = time.perf_counter() - t0
     return dom.n * steps / dt / 1e6, dt
+
+
+_COPY_PROBE = """
+import sys, torch
+n = int(sys.argv[1]) // 8
+a = torch.ones(n, dtype=torch.float64, device="cuda")
+b = torch.empty_like(a)
+for _ in range(3):
+    b.copy_(a)
+torch.cuda.synchronize()
+e0 = torch.cuda.Event(enable_timing=True)
+e1 = torch.cuda.Event(enable_timing=True)
+e0.record()
+for _ in range(20):
+    b.copy_(a)
+e1.record()
+torch.cuda.synchronize()
+print(2 * n * 8 * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+"""
+
+
+def copy_ceiling(bytes_each_way: int):
+    """Achievable HBM rate (read + write GB/s) of a plain device copy of
+    `bytes_each_way`, measured in a child process: torch carries its own HIP
+    runtime, which is kept out of this process."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, "-c", _COPY_PROBE, str(int(bytes_each_way))],
+                           capture_output=True, text=True, timeout=400)
+        return float(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (OSError, ValueError, IndexError, subprocess.TimeoutExpired):
+        return None
+
+
+def time_to_periodic(work, storage: str, layout: str, tol: float = 1e-4,
+                     max_cycles: int = 12) -> dict:
+    """Wall time of the plain run from rest until the cycle-to-cycle change
+    max |u(t) - u(t - T)| / max |u| drops below tol (the periodic-state
+    monitor), one cycle of the inlet period per check."""
+    lbm = work.solver(storage=storage, layout=layout)
+    lbm.synchronize()
+    period = int(round(work.period))
+    t0 = time.perf_counter()
+    lbm.snapshot_velocity()
+    change, cycles = float("inf"), 0
+    while cycles < max_cycles and not change < tol:
+        lbm.step(period)
+        cycles += 1
+        umax = lbm.monitor()["velocity_max"]
+        change = lbm.velocity_change(max(umax, 1e-30))
+        lbm.snapshot_velocity()
+    lbm.synchronize()
+    wall = time.perf_counter() - t0
+    return {"cycles": cycles if change < tol else None, "steps": cycles * period,
+            "period": period, "wall_s": round(wall, 3), "last_change": change, "tol": tol}
 
 
 def timed_rate(lbm, n_sites, steps, warmup):
@@ -179,6 +238,10 @@ def main() -> int:
     ap.add_argument("--report-steps", type=int, default=0,
                     help="steps per precision-report run (default: the workload's)")
     ap.add_argument("--report-workloads", default="")
+    ap.add_argument("--no-periodic", action="store_true",
+                    help="skip timing the run from rest to the periodic state")
+    ap.add_argument("--no-copy-ceiling", action="store_true",
+                    help="skip measuring the plain device-copy rate")
     ap.add_argument("--ensemble", action="store_true",
                     help="time the batched 96-vessel ensemble instead (FP32 storage)")
     args = ap.parse_args()
@@ -260,6 +323,19 @@ def main() -> int:
             del other
             log(f"[bench] {st}: {f_mlups:.1f} MLUPS ({f_mlups / mlups:.2f}x), "
                 f"{f_ach:.0f} GB/s algorithmic")
+    periodic = None
+    if not args.no_periodic:
+        periodic = {}
+        for st in (args.storage, "fp32") if args.storage != "fp32" else ("fp32",):
+            periodic[st] = time_to_periodic(work, st, args.layout)
+            log(f"[bench] {st} from rest to the periodic state: {periodic[st]}")
+    if not args.no_copy_ceiling:
+        cc = copy_ceiling(b_alg * n // 2)
+        if cc:
+            roof["copy_ceiling"] = round(cc, 1)
+            roof["frac_of_copy"] = round(achieved / cc, 4)
+            log(f"[bench] device copy of the step's bytes: {cc:.0f} GB/s; the step reaches "
+                f"{achieved / cc:.1%} of it")
     cpu = None
     if not args.no_cpu_baseline:
         c_mlups, c_s = cpu_baseline(dom, tau, inlets, outlets, args.cpu_steps)
@@ -281,6 +357,7 @@ def main() -> int:
                    **{k: round(v, 6) for k, v in work.meta.items()}},
         "roofline": roof,
         "forms": forms,
+        "periodic_state": periodic,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
