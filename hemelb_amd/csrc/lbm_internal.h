@@ -42,6 +42,10 @@ struct DevDomain {
   const int32_t* b_io = nullptr;      // [nb] iolet-site record or -1
   const float* b_q = nullptr;         // [nb][QS] cut distances of links 1..Q-1,
                                       // QS = Q-1 rounded up to a multiple of 4
+                                      // (binary64 storage)
+  const uint16_t* b_q16 = nullptr;    // [nb][QS16] the same as q * 65535,
+                                      // QS16 = Q-1 rounded up to a multiple of 8
+                                      // (reduced-precision storage)
   int64_t ni = 0;                  // iolet-site records
   const int32_t* io_iolet = nullptr;  // [ni] global iolet index (inlets first)
   const int32_t* io_bc = nullptr;     // [ni] hlb_iolet_bc of the site's group

@@ -638,19 +638,35 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
   if (!r.cut) return;
   const uint32_t rec = r.rec;
   const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
-  // cut distances of links 1..Q-1, padded to a multiple of 4 floats
+  // cut distances of links 1..Q-1: binary32 records of QS floats for
+  // binary64 storage, 16-bit fixed-point records of QS16 for the shifted forms
   constexpr int QS = (Q - 1 + 3) / 4 * 4;
-  float q[QS];
+  constexpr int QS16 = (Q - 1 + 7) / 8 * 8;
+  float q[QS16 > QS ? QS16 : QS];
   if (A.wall == 1 && wm) {
-    const float4* q4 = reinterpret_cast<const float4*>(D.b_q + (int64_t)rec * QS);
-    sfor<0, QS / 4>([&](auto ii) {
-      constexpr int i = decltype(ii)::value;
-      const float4 v = q4[i];
-      q[4 * i] = v.x;
-      q[4 * i + 1] = v.y;
-      q[4 * i + 2] = v.z;
-      q[4 * i + 3] = v.w;
-    });
+    if constexpr (kShifted<S>) {
+      const uint4* q8 = reinterpret_cast<const uint4*>(D.b_q16 + (int64_t)rec * QS16);
+      sfor<0, QS16 / 8>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const uint4 v = q8[i];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        sfor<0, 4>([&](auto jj) {
+          constexpr int j = decltype(jj)::value;
+          q[8 * i + 2 * j] = (float)(w[j] & 0xFFFFu) * (1.0f / 65535.0f);
+          q[8 * i + 2 * j + 1] = (float)(w[j] >> 16) * (1.0f / 65535.0f);
+        });
+      });
+    } else {
+      const float4* q4 = reinterpret_cast<const float4*>(D.b_q + (int64_t)rec * QS);
+      sfor<0, QS / 4>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        const float4 v = q4[i];
+        q[4 * i] = v.x;
+        q[4 * i + 1] = v.y;
+        q[4 * i + 2] = v.z;
+        q[4 * i + 3] = v.w;
+      });
+    }
     // BouzidiFirdaousLallemandLink::StreamLink (BouzidiFirdaousLallemand.h:41-66):
     // with q >= 0.5 and a fluid opposite link the slot gets
     // (f*_d + (2q - 1) f*_inv(d)) / (2q); both values are the site's own.

@@ -234,6 +234,7 @@ struct hlb_solver {
   DevBuf<uint32_t> binfo, b_wall, b_iolet;
   DevBuf<int32_t> b_io, io_iolet, io_bc, io_site, iolet_kind;
   DevBuf<float> b_q, iolet_nf;
+  DevBuf<uint16_t> b_q16;
   DevBuf<double> io_stash, io_ladd, iolet_n, weights;
   DevBuf<int32_t> bnbr;
   DevBuf<uint64_t> wave_fluid, wave_bnd;
@@ -296,7 +297,7 @@ struct hlb_solver {
     int64_t b = 2 * (int64_t)storage_bytes(p.storage) * Q * np;
     b += src.bytes() + binfo.bytes() + b_wall.bytes() + b_iolet.bytes() +
          b_io.bytes() + io_iolet.bytes() + io_bc.bytes() + io_site.bytes() +
-         iolet_kind.bytes() + b_q.bytes() + iolet_nf.bytes() + io_stash.bytes() +
+         iolet_kind.bytes() + b_q.bytes() + b_q16.bytes() + iolet_nf.bytes() + io_stash.bytes() +
          io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
          bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() +
          wave_rec.bytes() +
@@ -592,6 +593,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   std::vector<uint32_t> h_bwall, h_biolet;
   std::vector<int32_t> h_bio, h_io_iolet, h_io_bc, h_io_site;
   std::vector<float> h_bq;
+  std::vector<uint16_t> h_bq16;  // reduced-precision storage: q * 65535
   std::vector<double> h_ladd;
   for (int64_t o = 0; o < n; ++o) {
     const int64_t i = s->perm[o];
@@ -629,10 +631,21 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
       if (!blocks_layout) h_binfo[o] = rec + 1;
       h_bwall.push_back(wm);
       h_biolet.push_back(im);
-      const int qs = (Q - 1 + 3) / 4 * 4;  // record stride: 16-byte aligned
-      for (int d = 1; d <= qs; ++d)
-        h_bq.push_back(d < Q && dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1]
-                                                 : -1.0f);
+      // Cut distances of links 1..Q-1 in 16-byte aligned records: binary32
+      // for binary64 storage (the reference's float q, bit for bit);
+      // q * 65535 rounded to 16 bits for the reduced-precision forms (a wall
+      // position error below 8e-6 lattice units, half the bytes).
+      const bool reduced = P.storage != HLB_STORE_FP64;
+      const int qs = reduced ? (Q - 1 + 7) / 8 * 8 : (Q - 1 + 3) / 4 * 4;
+      for (int d = 1; d <= qs; ++d) {
+        const float q = d < Q && dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1]
+                                                  : -1.0f;
+        if (reduced)
+          h_bq16.push_back((uint16_t)std::lround((double)std::min(1.0f, std::max(0.0f, q)) *
+                                                 65535.0));
+        else
+          h_bq.push_back(q);
+      }
       if (im) {
         const int g = dm->site_type[i] == HLB_SITE_OUTLET ? 1 : 0;
         const int k = (g ? n_inlets : 0) + dm->iolet_id[i];
@@ -822,6 +835,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   HIP_OR_FAIL(s->b_iolet.upload(h_biolet));
   HIP_OR_FAIL(s->b_io.upload(h_bio));
   HIP_OR_FAIL(s->b_q.upload(h_bq));
+  HIP_OR_FAIL(s->b_q16.upload(h_bq16));
   HIP_OR_FAIL(s->io_iolet.upload(h_io_iolet));
   HIP_OR_FAIL(s->io_bc.upload(h_io_bc));
   HIP_OR_FAIL(s->io_site.upload(h_io_site));
@@ -876,6 +890,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   D.b_iolet = s->b_iolet.p;
   D.b_io = s->b_io.p;
   D.b_q = s->b_q.p;
+  D.b_q16 = s->b_q16.p;
   D.ni = (int64_t)h_io_iolet.size();
   D.io_iolet = s->io_iolet.p;
   D.io_bc = s->io_bc.p;

@@ -64,8 +64,9 @@ enum hlb_iolet_kind { HLB_IOLET_COSINE = 0, HLB_IOLET_PARABOLIC = 1,
 enum hlb_site_type { HLB_SITE_FLUID = 1, HLB_SITE_INLET = 2, HLB_SITE_OUTLET = 3 };
 /* Storage of the distributions in HBM.  HLB_STORE_FP64 is the plain method
  * (all arithmetic in binary64, bit-compatible with the reference).  FP32 and
- * FP16 store f_i - w_i*rho0 and collide in binary32; iolet boundary values
- * and all reductions stay binary64. */
+ * FP16 store f_i - w_i*rho0 and collide in binary32, with BFL cut distances
+ * held as 16-bit fixed point (q*65535); iolet boundary values and all
+ * reductions stay binary64. */
 enum hlb_storage { HLB_STORE_FP64 = 0, HLB_STORE_FP32 = 1, HLB_STORE_FP16 = 2 };
 /* Layout of the distributions in HBM.  HLB_LAYOUT_BLOCKS gives every 8^3
  * block that holds fluid 512 slots (Morton block order, z fastest) and finds
