@@ -87,6 +87,26 @@ def test_inconsistent_geometry_is_rejected(case):
     assert rc == N.HLB_E_INVALID and "iolet id" in err
 
 
+def test_empty_and_missing_site_arrays_are_rejected(case):
+    p = N.hlb_params()
+    p.lattice, p.kernel, p.wall = case.domain.Q, N.HLB_LBGK, N.HLB_WALL_SBB
+    p.inlet_bc = p.outlet_bc = N.HLB_IOLET_NASH
+    p.storage, p.tau, p.rho0, p.device, p.layout = 0, 0.8, 1.0, 0, 0
+    ins = L._iolet_array([i.to_product() for i in case.inlets])
+    outs = L._iolet_array([o.to_product() for o in case.outlets])
+    for n, drop in ((0, None), (-3, None), (case.domain.n, "coords"),
+                    (case.domain.n, "wall_mask")):
+        cd = case.domain.as_c()
+        cd.n_sites = n
+        if drop:
+            setattr(cd, drop, None)
+        h = C.c_void_p()
+        rc = N.lib().hlb_create(C.byref(p), C.byref(cd), ins, len(case.inlets), outs,
+                                len(case.outlets), C.byref(h))
+        assert rc == N.HLB_E_INVALID, (n, drop)
+        assert not h.value
+
+
 def test_ladd_needs_a_velocity_iolet(case):
     rc, err = _create(case, inlet_bc=N.HLB_IOLET_LADD)
     assert rc == N.HLB_E_INVALID and "velocity" in err
