@@ -7,6 +7,8 @@
   ``csrc/unet.cpp`` -> g++ (host code; the same compiler runtime and libm as the reference's own
   g++ build).
 * linked with g++ against libamdhip64 and zlib.
+* ``driver/hemelb_driver.cpp`` -> ``hemelb_amd/hemelb_driver``: the C++
+  driver written against the header-only layer ``include/hemelb_amd.hpp``.
 
 Usage: ``python -m hemelb_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -23,6 +25,7 @@ BUILD = os.path.join(PKG, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 LIB = os.path.join(PKG, "libhemelb_amd.so")
+DRIVER = os.path.join(PKG, "hemelb_driver")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -66,6 +69,13 @@ def build() -> str:
          ["-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-lz",
           "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
+    # the C++ driver written against include/hemelb_amd.hpp
+    tmpd = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE,
+          os.path.join(PKG, "driver", "hemelb_driver.cpp"), "-o", tmpd, "-L", PKG,
+          "-lhemelb_amd", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath-link," + os.path.join(ROCM, "lib")])
+    os.replace(tmpd, DRIVER)
     return LIB
 
 
