@@ -193,6 +193,6 @@ def declared_functions(header_paths) -> list[str]:
     for p in header_paths:
         txt = open(p).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(hlb[gx]?_\w+)\s*\(", txt, flags=re.M):
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(hlb[gxn]?_\w+)\s*\(", txt, flags=re.M):
             names.append(m.group(1))
     return sorted(set(names))
