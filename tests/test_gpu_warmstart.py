@@ -1,0 +1,53 @@
+"""What the trained network gains over the plain method.
+
+The warm-start pipeline runs in a child process (`python -m
+hemelb_amd.warmstart`). torch ships its own HIP runtime, which has to
+initialise before the library's, and this test process has already used
+the library.
+
+The run generates periodic states of 16 seeded vessels with the plain
+binary32 solver and trains the U-Net on them for 20 s. It then runs 8
+held-out vessels to the periodic state twice: from rest (the plain method)
+and from the network's prediction.
+
+The assertions:
+* the training loss falls;
+* the prediction is closer to the periodic field than the plain method's
+  starting state (relative error 1, the field itself);
+* both runs converge to the same periodic field.
+
+The measured cycle and wall-time ratios are printed. They are reported, not
+asserted: at this training budget they are close to 1.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.timeout(900)
+def test_trained_network_beats_the_plain_start(tmp_path):
+    out = str(tmp_path / "ws.json")
+    cmd = [sys.executable, "-u", "-m", "hemelb_amd.warmstart", "--side", "64", "--regime", "slow",
+           "--train-runs", "16", "--batch-runs", "16", "--eval-runs", "8",
+           "--periods", "400", "--eval-period", "400", "--epochs", "200",
+           "--train-seconds", "20", "--max-cycles", "80", "--out", out]
+    print("[warmstart] " + " ".join(cmd[3:]), flush=True)
+    # the child's progress goes to this process's (captured) output
+    r = subprocess.run(cmd, cwd=ROOT, timeout=840)
+    assert r.returncode == 0
+    res = json.load(open(out))
+    print(f"[warmstart] prediction error {res['prediction_rel_l2_median']:.3f}; cycles cold "
+          f"{sum(res['cycles_cold'])} warm {sum(res['cycles_warm'])} (ratio "
+          f"{res['cycle_ratio_total']:.2f}); wall {res['wall_cold_s']:.2f}s vs "
+          f"{res['wall_warm_s']:.2f}s", flush=True)
+    assert res["losses"][-1] < res["losses"][0]
+    assert not res["unstable_cold"] and not res["unstable_warm"]
+    assert res["prediction_rel_l2_median"] < 1.0
+    assert res["converged_field_median_rel_diff"] < 1e-4
