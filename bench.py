@@ -312,7 +312,7 @@ def main() -> int:
             if st == args.storage:
                 continue
             other = work.solver(storage=st, layout=args.layout)
-            if launch:
+            if launch and (args.threads or args.xcd_remap >= 0):
                 other.set_launch_shape(**launch)
             f_ms, f_mlups = timed_rate(other, n, args.form_steps, args.warmup)
             f_ach = 2 * 19 * STORE_BYTES[st] * n / (f_ms * 1e-3) / 1e9

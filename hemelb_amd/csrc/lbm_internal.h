@@ -14,10 +14,12 @@ constexpr int kMaxQ = 19;
 // Device-side description of one solver's domain in the library's own
 // layout.  Sites are ordered by 8^3 block in Morton order, z-fastest inside a
 // block ("compact" order).  Distributions are structure-of-arrays with stride
-// `np`, indexed either by compact site (layout 0, with a 32-bit pull table)
-// or by slot in dense 512-slot blocks (layout 1, neighbours by arithmetic).
+// `np`, indexed by compact site with a 32-bit pull table (layout 0), by slot
+// in dense 512-slot blocks with neighbours by arithmetic (layout 1), or by
+// compact site with neighbours from the blocks' per-plane fluid masks
+// (layout 2).
 struct DevDomain {
-  int layout = 1;   // 0 = compact + pull table, 1 = dense blocks
+  int layout = 1;   // 0 = pull table, 1 = dense blocks, 2 = compact blocks
   int64_t n = 0;    // fluid sites
   int64_t np = 0;   // stride of every per-direction state array
   int64_t npc = 0;  // stride of compact binary64 exports (n padded)
@@ -84,9 +86,9 @@ struct StepArgs {
   // (0: one pass over the domain; fewer: a persistent grid that strides over
   // it) and the workgroup-to-XCD remap (0: hardware round-robin order; 1:
   // one contiguous range per XCD; K = 2^k: runs of K workgroups per XCD,
-  // the eight XCDs on adjacent runs).  Default: 128 threads in runs of 8
-  // workgroups (four 8^3 blocks per run), which keeps a block's planes and
-  // most of its Morton neighbours in one XCD's L2 while the whole chip
+  // the eight XCDs on adjacent runs).  Default (set per storage by the
+  // solver): 128 threads in runs of 8 workgroups (two 8^3 blocks per run),
+  // which keeps a block's planes in one XCD's L2 while the whole chip
   // sweeps one region of the domain; one range per XCD measured 8-12 %
   // slower on the vessel tree.
   uint32_t threads = 128;

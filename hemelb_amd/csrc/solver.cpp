@@ -916,6 +916,12 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   A.unstable = s->unstable.p;
   // MRT relaxation rate of the viscous moments (DHumieres*MRTBasis.cc)
   A.mrt_s_tau = 1.0 / P.tau;
+  // Default launch shape: 128-thread workgroups; runs of 8 workgroups per
+  // XCD for binary64 and binary32 storage, the hardware order for binary16
+  // (whose planes are one cache line each; measured 2 % faster on the
+  // vessel tree).
+  A.threads = 128;
+  A.xcd_remap = P.storage == HLB_STORE_FP16 ? 0u : 8u;
   s->initialised = false;
   *out = s.release();
   return HLB_OK;

@@ -174,7 +174,9 @@ int hlb_set_distributions(hlb_solver* s, const double* f_in);
  * Lattice::CalculateFeq (Lattice.h:236-320): the equilibrium initial
  * condition of InitialCondition.hpp:39-52 generalised from one uniform
  * value to a field (e.g. a predicted flow field).  `device` != 0: the
- * arrays are device pointers.  The time is left unchanged. */
+ * arrays are device pointers, read on the solver's stream; the work that
+ * writes them must be complete before the call.  The time is left
+ * unchanged. */
 int hlb_set_equilibrium_fields(hlb_solver* s, const double* density, const double* momentum,
                                int32_t device);
 
