@@ -60,6 +60,8 @@ _SIGS = {
     "orc_set_time": (None, [_V, C.c_uint64]),
     "orc_get_time": (C.c_uint64, [_V]),
     "orc_step": (C.c_int, [_V, C.c_int64]),
+    "orc_threads": (C.c_int, []),
+    "orc_set_threads": (None, [C.c_int]),
     "orc_get_f": (None, [_V, _V]),
     "orc_set_f": (None, [_V, _V]),
     "orc_macro": (None, [_V, _V, _V]),

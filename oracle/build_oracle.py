@@ -1,6 +1,7 @@
 """Build recipe for the CPU oracle (test infrastructure).
 
-* ``oracle/liboracle.so`` -- the C restatement in ``hlb_oracle.c``; always built.
+* ``oracle/liboracle.so`` -- the C restatement in ``hlb_oracle.c`` (OpenMP over
+  sites in the step); always built.
 * ``oracle/_ref/libhlbref.so`` -- ``ref_glue.cpp`` compiled together with the
   reference's own ``Code/util/Bessel.cc``, ``Code/util/UnitConverter.cc``, ``Code/util/Matrix3D.cc`` and
   ``Code/lb/kernels/DHumieresD3Q{15,19}MRTBasis.cc`` where they lie under
@@ -29,7 +30,8 @@ def _run(cmd):
 def build_oracle() -> str:
     out = os.path.join(HERE, "liboracle.so")
     _run(["gcc", "-std=c11", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
-          "-fno-fast-math", "-Wall", "-o", out, os.path.join(HERE, "hlb_oracle.c"), "-lm"])
+          "-fno-fast-math", "-fopenmp", "-Wall", "-o", out, os.path.join(HERE, "hlb_oracle.c"),
+          "-lm"])
     return out
 
 

@@ -19,6 +19,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #define ORC_PI 3.14159265358979323846264338327950288 /* constants.h:18 */
 #define ORC_CS2 (1.0 / 3.0)                         /* constants.h:41 */
 #define ORC_MMHG_TO_PA 133.3223874                  /* constants.h:20 */
@@ -803,6 +807,23 @@ static void post_step_site(orc_sim* s, int64_t i) {
   }
 }
 
+/* Host threads the oracle's step uses (OpenMP; 1 when built without it). */
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* SimulationMaster::DoTimeStep (SimulationMaster.impl.h:168-220) reduced to
  * the LB actor: PreSend/PreReceive stream-and-collide every site, PostReceive
  * runs PostStep over every site, then SwapOldAndNew and Increment.  The
@@ -810,15 +831,20 @@ static void post_step_site(orc_sim* s, int64_t i) {
 int orc_step(orc_sim* s, int64_t nsteps) {
   int unstable = 0;
   const int Q = s->L.Q;
+  const int64_t n = s->n;
   for (int64_t k = 0; k < nsteps; ++k) {
-    for (int64_t i = 0; i < s->n; ++i) stream_and_collide_site(s, i);
-    for (int64_t i = 0; i < s->n; ++i) post_step_site(s, i);
+    /* Sites are independent within each phase: every fNew slot has exactly
+     * one writer in the push scheme and PostStep touches only the site's own
+     * slots, so the host threads split the sites without changing any
+     * site's arithmetic. */
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) stream_and_collide_site(s, i);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) post_step_site(s, i);
     unstable = 0;
-    for (int64_t i = 0; i < s->n * Q; ++i)
-      if (!(s->f_new[i] > 0.0)) {
-        unstable = 1;
-        break;
-      }
+#pragma omp parallel for schedule(static) reduction(| : unstable)
+    for (int64_t i = 0; i < n * Q; ++i)
+      if (!(s->f_new[i] > 0.0)) unstable |= 1;
     double* tmp = s->f_old;
     s->f_old = s->f_new;
     s->f_new = tmp;

@@ -101,6 +101,9 @@ uint64_t orc_get_time(const orc_sim* s);
 /* Runs n timesteps; returns 1 if the StabilityTester would flag the last
  * step unstable (some fNew not > 0), else 0. */
 int orc_step(orc_sim* s, int64_t n);
+/* Host threads of orc_step (OpenMP over sites; results do not depend on it). */
+int orc_threads(void);
+void orc_set_threads(int n);
 void orc_get_f(const orc_sim* s, double* out);
 void orc_set_f(orc_sim* s, const double* in);
 void orc_macro(const orc_sim* s, double* rho, double* u);

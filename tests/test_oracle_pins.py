@@ -551,3 +551,21 @@ def test_stress_outputs_of_a_simple_shear_flow(Q):
         pi_xy = -tau * rho * gamma / 3.0
         sp = (1 - 1 / (2 * tau)) / np.sqrt(2)
         assert abs(vm - sp * np.sqrt(6.0) * abs(pi_xy)) <= 1e-12 * abs(vm)
+
+
+def test_threaded_step_equals_single_thread():
+    """The oracle's step splits the sites over host threads; every fNew slot
+    has one writer, so the result must not depend on the thread count
+    (bit for bit, BFL PostStep and Ladd/Nash iolets included)."""
+    case = K.tree_case(3, 6.0, seed=1)
+    n0 = O.lib().orc_threads()
+    try:
+        O.lib().orc_set_threads(1)
+        a = case.oracle()
+        a.step(6)
+        O.lib().orc_set_threads(max(2, n0))
+        b = case.oracle()
+        b.step(6)
+    finally:
+        O.lib().orc_set_threads(n0)
+    np.testing.assert_array_equal(a.get_f(), b.get_f())
