@@ -5,6 +5,7 @@
 this package mirrors the reference's ``hemelb::lb`` names on the host.
 """
 from . import _native  # noqa: F401
+from .config import SimBuilder, SimConfig, read_xml, write_xml  # noqa: F401
 from .geometry import Domain, Geometry, IoletPlane  # noqa: F401
 from .lb import (D3Q15, D3Q19, LBGK, LBM, MRT, BouzidiFirdaousLallemand,  # noqa: F401
                  InOutLetCosine, InOutLetParabolicVelocity, InOutLetWomersleyVelocity,
@@ -16,4 +17,5 @@ __all__ = [
     "SimpleBounceBack", "BouzidiFirdaousLallemand", "NashZerothOrderPressure",
     "LaddIolet", "InOutLetCosine", "InOutLetParabolicVelocity",
     "InOutLetWomersleyVelocity", "LbmParameters", "Traits", "UnitConverter",
+    "SimConfig", "SimBuilder", "read_xml", "write_xml",
 ]

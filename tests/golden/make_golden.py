@@ -7,6 +7,11 @@ own tests hold (run once in a container that has the reference tree):
 * four_cube.gmy -- Code/tests/resources/four_cube.gmy, the geometry file that
   Code/tests/geometry/GeometryReaderTests.cc:41-76 compares with the
   programmatic FourCube domain.
+* four_cube.xml, config.xml, config-velocity-iolet.xml,
+  config_new_velocity_inlets.xml -- the settings files of
+  Code/tests/resources that Code/tests/configuration/SimConfigTests.cc and
+  Code/tests/lb/iolets/InOutLetTests.cc:60-200 read (their expected values
+  are restated in tests/test_config.py).
 * mrt_basis_diag.json -- BASIS_TIMES_BASIS_TRANSPOSED expected by
   Code/tests/lb/KernelTests.cc:358-370.
 * xtr_headers.json -- the main and field header bytes that
@@ -38,6 +43,9 @@ def main():
               open(os.path.join(HERE, "bessel_j0_mathematica.json"), "w"), indent=1)
     shutil.copyfile(os.path.join(REF, "resources", "four_cube.gmy"),
                     os.path.join(HERE, "four_cube.gmy"))
+    for name in ("four_cube.xml", "config.xml", "config-velocity-iolet.xml",
+                 "config_new_velocity_inlets.xml"):
+        shutil.copyfile(os.path.join(REF, "resources", name), os.path.join(HERE, name))
     kt = open(os.path.join(REF, "lb", "KernelTests.cc")).read()
     d15 = re.search(r"std::array<double, 11>\{([^}]*)\}", kt).group(1)
     d19 = re.search(r"std::array<double, 15>\{([^}]*)\}", kt).group(1)
