@@ -216,7 +216,8 @@ def ensemble_bench(args) -> int:
         "config": {"workload": "stenosed_ensemble_96_D3Q19_LBGK_BFL_Nash", "vessels": len(sizes),
                    "fluid_sites": n, "sites_per_vessel": [min(sizes), max(sizes)],
                    "storage": storage, "launch": {k: best[k] for k in
-                                                  ("threads", "planes", "groups", "xcd_remap")},
+                                                  ("threads", "planes", "groups", "xcd_remap",
+                                                   "split")},
                    "setup_s": round(setup_s, 2), "bytes_alg_per_site": b_alg},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
@@ -239,6 +240,9 @@ def main() -> int:
     ap.add_argument("--xcd-remap", type=int, default=-1,
                     help="workgroup-to-XCD remap (-1: the solver's default; 0 hardware "
                          "order, 1 one range per XCD, K>=2 runs of K workgroups per XCD)")
+    ap.add_argument("--split", type=int, default=-1,
+                    help="planes with and without boundary sites in two launches (1) or "
+                         "one (0); -1: the solver's default")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="oracle steps for the CPU baseline (0: about 10 s of them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -280,12 +284,13 @@ def main() -> int:
     setup_s = time.time() - t0
     launch = None
     if args.layout != "pull_table":
-        if args.threads or args.xcd_remap >= 0:
+        if args.threads or args.xcd_remap >= 0 or args.split >= 0:
             sh = lbm.launch_shape
             lbm.set_launch_shape(threads=args.threads or sh["threads"], planes=sh["planes"],
                                  groups=sh["groups"],
                                  xcd_remap=sh["xcd_remap"] if args.xcd_remap < 0
-                                 else args.xcd_remap)
+                                 else args.xcd_remap,
+                                 split=sh["split"] if args.split < 0 else args.split)
         launch = lbm.launch_shape
     log(f"[bench] solver set-up {setup_s:.1f}s, device memory "
         f"{lbm.device_bytes() / 2**30:.2f} GiB")
@@ -323,7 +328,7 @@ def main() -> int:
             if st == args.storage:
                 continue
             other = work.solver(storage=st, layout=args.layout)
-            if launch and (args.threads or args.xcd_remap >= 0):
+            if launch and (args.threads or args.xcd_remap >= 0 or args.split >= 0):
                 other.set_launch_shape(**launch)
             f_ms, f_mlups = timed_rate(other, n, args.form_steps, args.warmup)
             f_ach = 2 * 19 * STORE_BYTES[st] * n / (f_ms * 1e-3) / 1e9

@@ -30,6 +30,11 @@ struct DevDomain {
   const uint64_t* wave_bnd = nullptr;    // [nwaves] lane has a boundary record
   const uint32_t* wave_rec = nullptr;    // [nwaves] first boundary record
   const uint32_t* wave_cbase = nullptr;  // [nwaves] compact index of first fluid lane
+  // Block layouts: the planes holding fluid split into those without any
+  // boundary site and those with one (Morton order within each list).
+  const uint32_t* bulk_waves = nullptr;
+  const uint32_t* bnd_waves = nullptr;
+  uint32_t n_bulk_waves = 0, n_bnd_waves = 0;
   // Pull table (layout 0): src[(d-1)*np + x] is the offset inside array d
   // of the value streaming into slot d of site x: the site at x - c_d, or
   // x + (inv(d) - d) * np (the site's own inverse slot) for a cut link.
@@ -95,6 +100,14 @@ struct StepArgs {
   uint32_t planes = 1;
   uint32_t groups = 0;
   uint32_t xcd_remap = 8;
+  // One-pass shapes: run the planes without boundary sites and those with
+  // them as two launches of separately compiled kernels (1) or all planes
+  // in one launch (0).  The bulk kernel needs far fewer registers (FP64
+  // 86 instead of 116 VGPRs, FP32/FP16 46 instead of 76), but on the
+  // vessel tree the two serial launches measured 6-7 % slower for FP64 and
+  // FP32 and 1.4 % faster for FP16, so one launch is the default and the
+  // split is a tuner candidate.
+  uint32_t split = 0;
 };
 
 // Launches; every function returns 0 or a HIP error code.

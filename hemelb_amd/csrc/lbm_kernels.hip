@@ -624,7 +624,7 @@ __device__ __forceinline__ double ladd_correction(const DevDomain& D,
 // All Q loads are issued first: a cut link reads the site's own value of the
 // inverse direction (the bounce-back value); the wall and iolet formulas are
 // applied afterwards.
-template <int Q, typename S, typename T, class Lay>
+template <int Q, typename S, typename T, class Lay, bool BULK = false>
 __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
                                        const S* __restrict__ fin, const SiteRef& r,
                                        T (&f)[Q]) {
@@ -635,6 +635,7 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
     constexpr int e = decltype(ei)::value;
     f[e] = St::ld(fin + (int64_t)e * np + Lay::template offset<Q, e>(D, r));
   });
+  if constexpr (BULK) return;  // a wave without boundary sites
   if (!r.cut) return;
   const uint32_t rec = r.rec;
   const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
@@ -719,9 +720,10 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
   }
 }
 
-template <int Q, typename T>
+template <int Q, typename T, bool BULK = false>
 __device__ __forceinline__ void update_stash(const DevDomain& D, const SiteRef& r,
                                              const Hydro<Q, T>& h) {
+  if constexpr (BULK) return;
   if (!r.cut) return;
   const int io = D.b_io[r.rec];
   if (io < 0) return;
@@ -746,13 +748,13 @@ __device__ __forceinline__ void store_out(S* p, typename Store<S>::T v) {
   __builtin_nontemporal_store((S)v, p);
 }
 
-template <int Q, int KERNEL, typename S, class Lay>
+template <int Q, int KERNEL, typename S, class Lay, bool BULK = false>
 __device__ __forceinline__ void step_site(const DevDomain& D, const StepArgs& A,
                                           const S* __restrict__ fin,
                                           S* __restrict__ fout, const SiteRef& r) {
   using T = typename Store<S>::T;
   T f[Q];
-  gather<Q, S, T, Lay>(D, A, fin, r, f);
+  gather<Q, S, T, Lay, BULK>(D, A, fin, r, f);
   bool ok = true;
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
@@ -765,7 +767,7 @@ __device__ __forceinline__ void step_site(const DevDomain& D, const StepArgs& A,
   StepArgs a = A;
   Lay::relax(D, r, a);
   const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, a);
-  update_stash<Q, T>(D, r, h);
+  update_stash<Q, T, BULK>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
     store_out<S>(fout + (int64_t)d * D.np + r.x, f[d]);
@@ -787,6 +789,27 @@ __global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
     if (!Lay::locate(D, r)) return;
   }
   step_site<Q, KERNEL, S, Lay>(D, A, fin, fout, r);
+}
+
+// The timestep over a list of waves (x planes).  The solver splits the
+// planes of the block layouts into those without any boundary site, run by
+// the BULK instantiation (no boundary code, so far fewer registers and more
+// waves in flight), and those with one, run by the full one; together the
+// two launches update every site exactly as k_step does.
+template <int Q, int KERNEL, typename S, class Lay, bool BULK>
+__global__ __launch_bounds__(256) void k_step_list(DevDomain D, StepArgs A,
+                                                   const S* __restrict__ fin,
+                                                   S* __restrict__ fout,
+                                                   const uint32_t* __restrict__ waves,
+                                                   uint32_t count) {
+  const uint32_t g = step_workgroup(A.xcd_remap);
+  const uint32_t i = __builtin_amdgcn_readfirstlane(g * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (i >= count) return;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(waves[i]);
+  SiteRef r;
+  if (!Lay::locate_wave(D, w, r)) return;
+  if constexpr (BULK) r.cut = 0;  // known: no cut links in these planes
+  step_site<Q, KERNEL, S, Lay, BULK>(D, A, fin, fout, r);
 }
 
 // The same timestep for the block layout with a wave loop: each wave takes
@@ -1326,6 +1349,29 @@ int launch_step(const DevDomain& d, const StepArgs& a, const void* fin, void* fo
                             ((uint64_t)wpg * a.planes);
       uint64_t groups = a.groups ? std::min<uint64_t>(a.groups, need) : need;
       const bool one_pass = a.planes == 1 && groups == need;
+      if (one_pass && a.split && d.bulk_waves && d.bnd_waves) {
+        // bulk planes first, then the planes with boundary sites
+        const uint32_t counts[2] = {d.n_bulk_waves, d.n_bnd_waves};
+        const uint32_t* lists[2] = {d.bulk_waves, d.bnd_waves};
+        for (int part = 0; part < 2; ++part) {
+          if (!counts[part]) continue;
+          uint64_t gp = ((uint64_t)counts[part] + wpg - 1) / wpg;
+          if (a.xcd_remap >= 2) {
+            const uint64_t m = 8ull * a.xcd_remap;
+            gp = (gp + m - 1) / m * m;
+          }
+          const dim3 gridp((unsigned)gp), blockp(a.threads);
+          if (part == 0)
+            hipLaunchKernelGGL((k_step_list<decltype(q)::value, decltype(k)::value, S, Lay, true>),
+                               gridp, blockp, 0, (hipStream_t)stream, d, a, (const S*)fin,
+                               (S*)fout, lists[part], counts[part]);
+          else
+            hipLaunchKernelGGL((k_step_list<decltype(q)::value, decltype(k)::value, S, Lay, false>),
+                               gridp, blockp, 0, (hipStream_t)stream, d, a, (const S*)fin,
+                               (S*)fout, lists[part], counts[part]);
+        }
+        return (int)hipGetLastError();
+      }
       if (a.xcd_remap >= 2) {
         // runs of a.xcd_remap workgroups per XCD: pad the grid to whole
         // groups of 8 runs; padded workgroups see planes >= nwaves and exit

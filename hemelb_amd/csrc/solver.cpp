@@ -238,7 +238,7 @@ struct hlb_solver {
   DevBuf<double> io_stash, io_ladd, iolet_n, weights;
   DevBuf<int32_t> bnbr;
   DevBuf<uint64_t> wave_fluid, wave_bnd;
-  DevBuf<uint32_t> wave_rec, wave_cbase;
+  DevBuf<uint32_t> wave_rec, wave_cbase, bulk_waves, bnd_waves;
   std::vector<int64_t> slot_of;   // compact -> state index (slot or compact)
   DevBuf<int64_t> d_perm;
   void* state[2] = {nullptr, nullptr};
@@ -301,7 +301,7 @@ struct hlb_solver {
          io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
          bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() +
          wave_rec.bytes() +
-         wave_cbase.bytes() +
+         wave_cbase.bytes() + bulk_waves.bytes() + bnd_waves.bytes() +
          s_soa.bytes() + s_aos.bytes() + rho.bytes() + u.bytes() + u_snap.bytes();
     return b;
   }
@@ -683,6 +683,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   // Block layout tables: neighbour blocks and per-wave masks.
   std::vector<int32_t> h_bnbr;
   std::vector<uint64_t> h_wf, h_wb;
+  std::vector<uint32_t> h_bulkw, h_bndw;
   std::vector<uint32_t> h_wr, h_wc;
   if (blocks_layout) {
     // Block neighbour table (offsets (dx,dy,dz) in {-1,0,1}^3, index 13 = self)
@@ -734,6 +735,9 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
       }
     }
     s->dom.nwaves = (uint32_t)nwaves;
+    // planes without boundary sites and planes with them, for the split step
+    for (int64_t w = 0; w < nwaves; ++w)
+      if (h_wf[w]) (h_wb[w] ? h_bndw : h_bulkw).push_back((uint32_t)w);
     // Check, on the host, every neighbour the block-layout kernel will
     // compute (BlockLayout::src in lbm_kernels.hip): each bulk pull of each
     // fluid site must land on the slot of its source site.
@@ -831,6 +835,8 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   HIP_OR_FAIL(s->wave_bnd.upload(h_wb));
   HIP_OR_FAIL(s->wave_rec.upload(h_wr));
   HIP_OR_FAIL(s->wave_cbase.upload(h_wc));
+  HIP_OR_FAIL(s->bulk_waves.upload(h_bulkw));
+  HIP_OR_FAIL(s->bnd_waves.upload(h_bndw));
   HIP_OR_FAIL(s->b_wall.upload(h_bwall));
   HIP_OR_FAIL(s->b_iolet.upload(h_biolet));
   HIP_OR_FAIL(s->b_io.upload(h_bio));
@@ -883,6 +889,12 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   D.wave_bnd = s->wave_bnd.p;
   D.wave_rec = s->wave_rec.p;
   D.wave_cbase = s->wave_cbase.p;
+  if (blocks_layout) {
+    D.bulk_waves = s->bulk_waves.p;
+    D.bnd_waves = s->bnd_waves.p;
+    D.n_bulk_waves = (uint32_t)h_bulkw.size();
+    D.n_bnd_waves = (uint32_t)h_bndw.size();
+  }
   D.src = s->src.p;
   D.binfo = s->binfo.p;
   D.nb = (int64_t)h_bwall.size();
@@ -1413,6 +1425,7 @@ static int check_shape(const hlb_solver* s, const hlb_launch_shape& sh) {
   if (sh.groups < 0) return fail(HLB_E_INVALID, "workgroup count must not be negative");
   if (sh.xcd_remap < 0 || sh.xcd_remap > 4096)
     return fail(HLB_E_INVALID, "xcd_remap must be in 0..4096");
+  if (sh.split != 0 && sh.split != 1) return fail(HLB_E_INVALID, "split must be 0 or 1");
   return HLB_OK;
 }
 
@@ -1424,6 +1437,7 @@ int hlb_set_launch_shape(hlb_solver* s, const hlb_launch_shape* shape) {
   s->args.planes = (uint32_t)shape->planes;
   s->args.groups = (uint32_t)shape->groups;
   s->args.xcd_remap = (uint32_t)shape->xcd_remap;
+  s->args.split = (uint32_t)shape->split;
   return HLB_OK;
 }
 
@@ -1433,6 +1447,7 @@ int hlb_get_launch_shape(const hlb_solver* s, hlb_launch_shape* shape) {
   shape->planes = (int32_t)s->args.planes;
   shape->groups = (int32_t)s->args.groups;
   shape->xcd_remap = (int32_t)s->args.xcd_remap;
+  shape->split = (int32_t)s->args.split;
   return HLB_OK;
 }
 
@@ -1463,15 +1478,17 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* b
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->p.device);
   std::vector<hlb_launch_shape> cand;
-  for (int th : {256, 128, 64})
-    for (int xcd : {1, 0}) cand.push_back({th, 1, 0, xcd});
-  for (int run : {2, 16, 64}) cand.push_back({256, 1, 0, run});
-  cand.push_back({128, 1, 0, 4});
-  cand.push_back({128, 1, 0, 8});
-  cand.push_back({256, 2, 0, 1});
-  cand.push_back({128, 2, 0, 1});
-  for (int th : {256, 128}) cand.push_back({th, 1, cus * 24 * 64 / th, 1});
-  cand.push_back({256, 2, cus * 24 * 64 / 256, 1});
+  for (int split : {1, 0}) {
+    for (int th : {256, 128, 64})
+      for (int xcd : {1, 0}) cand.push_back({th, 1, 0, xcd, split});
+    for (int run : {2, 16, 64}) cand.push_back({256, 1, 0, run, split});
+    cand.push_back({128, 1, 0, 4, split});
+    cand.push_back({128, 1, 0, 8, split});
+  }
+  cand.push_back({256, 2, 0, 1, 0});
+  cand.push_back({128, 2, 0, 1, 0});
+  for (int th : {256, 128}) cand.push_back({th, 1, cus * 24 * 64 / th, 1, 0});
+  cand.push_back({256, 2, cus * 24 * 64 / 256, 1, 0});
   hlb_launch_shape keep{};
   hlb_get_launch_shape(s, &keep);
   double best_t = 1e300;

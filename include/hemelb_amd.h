@@ -248,6 +248,9 @@ typedef struct {
   int32_t planes;
   int32_t groups;
   int32_t xcd_remap;
+  int32_t split;       /* one-pass shapes: 1 runs the planes without boundary
+                          sites and those with them as two launches of
+                          separately compiled kernels, 0 one launch */
 } hlb_launch_shape;
 int hlb_set_launch_shape(hlb_solver* s, const hlb_launch_shape* shape);
 int hlb_get_launch_shape(const hlb_solver* s, hlb_launch_shape* shape);

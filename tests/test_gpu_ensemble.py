@@ -70,8 +70,9 @@ def test_launch_shapes_agree_bitwise(ensemble):
               dict(planes=3, groups=5), dict(xcd_remap=2), dict(xcd_remap=16),
               dict(threads=128, xcd_remap=64), dict(groups=9, xcd_remap=4),
               dict(planes=2, xcd_remap=8), dict(threads=64, xcd_remap=3),
-              dict(groups=11, xcd_remap=5)]
-    for bad in (dict(xcd_remap=4097), dict(xcd_remap=-1)):
+              dict(groups=11, xcd_remap=5), dict(threads=128, xcd_remap=8, split=0),
+              dict(threads=256, split=0), dict(threads=64, xcd_remap=16, split=1)]
+    for bad in (dict(xcd_remap=4097), dict(xcd_remap=-1), dict(split=2)):
         with pytest.raises(Exception):
             ref.set_launch_shape(**bad)
     for sh in shapes:
@@ -90,7 +91,8 @@ def test_autotune_restores_the_state(ensemble):
     print(f"[ensemble] autotuned shape {best}", flush=True)
     assert lbm.GetTimeStep() == t
     np.testing.assert_array_equal(lbm.get_distributions(), before)
-    assert lbm.launch_shape == {k: best[k] for k in ("threads", "planes", "groups", "xcd_remap")}
+    assert lbm.launch_shape == {k: best[k] for k in ("threads", "planes", "groups", "xcd_remap",
+                                                     "split")}
     other = e.solver(storage="fp32")
     other.step(5)
     lbm.step(10)

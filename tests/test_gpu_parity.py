@@ -269,3 +269,18 @@ def test_equilibrium_fields_reduced_precision():
         err = np.abs(g.get_distributions() - want).max() / np.abs(want).max()
         print(f"[parity] equilibrium fields {storage}: max rel error {err:.2e}", flush=True)
         assert err < tol
+
+
+@pytest.mark.parametrize("layout", ["blocks", "compact_blocks"])
+def test_split_launch_is_bitexact(layout):
+    """Planes with and without boundary sites in two launches of separately
+    compiled kernels give the oracle's results bit for bit."""
+    case = K.tree_case(3, 6.0, seed=1)
+    print(f"[parity] split launch layout={layout} n={case.domain.n}", flush=True)
+    orc = case.oracle()
+    gpu = case.product(layout=layout)
+    gpu.set_launch_shape(threads=128, xcd_remap=8, split=1)
+    for n in (1, 2, 17):
+        orc.step(n)
+        gpu.step(n)
+        np.testing.assert_array_equal(gpu.get_distributions(), orc.get_f())

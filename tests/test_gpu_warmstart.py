@@ -36,7 +36,7 @@ def test_trained_network_beats_the_plain_start(tmp_path):
     out = str(tmp_path / "ws.json")
     cmd = [sys.executable, "-u", "-m", "hemelb_amd.warmstart", "--side", "64", "--regime", "slow",
            "--train-runs", "16", "--batch-runs", "16", "--eval-runs", "8",
-           "--periods", "400", "--eval-period", "400", "--epochs", "200",
+           "--periods", "400", "--eval-period", "400", "--epochs", "200", "--lr", "3e-4",
            "--train-seconds", "20", "--max-cycles", "80", "--out", out]
     print("[warmstart] " + " ".join(cmd[3:]), flush=True)
     # the child's progress goes to this process's (captured) output
