@@ -384,24 +384,6 @@ def simulate(specs, side: int, period: int, phases: int = 8, tol: float = 1e-4,
     return targets, masks, cycles, _time.perf_counter() - t0
 
 
-def phase_of(ph: int, phases: int, start_step: int, period: int) -> float:
-    """Inlet phase (radians) of sample ph: the cosine iolet uses the 0-indexed
-    step t - 1 (InOutLetCosine.cc:26-43)."""
-    t = start_step + int(round(ph * period / phases)) - 1
-    return 2.0 * math.pi * (t % period) / period
-
-
-def equilibrium(rho: np.ndarray, u: np.ndarray, Q: int = 19) -> np.ndarray:
-    """f_eq(rho, rho u) per site ([n, Q]), the textbook compressible form."""
-    from . import lb as L
-    lat = L.LATTICES[Q]
-    c = np.asarray(lat.VECTORS, np.float64)
-    w = np.asarray(lat.EQMWEIGHTS, np.float64)
-    cu = u @ c.T
-    uu = (u * u).sum(1)[:, None]
-    return rho[:, None] * w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu)
-
-
 # ---------------------------------------------------------------------------
 # Training and the warm-start evaluation
 # ---------------------------------------------------------------------------

@@ -9,16 +9,6 @@ from hemelb_amd import lb as L
 from hemelb_amd import warmstart as WS
 
 
-def test_equilibrium_has_the_requested_moments():
-    rng = np.random.default_rng(0)
-    rho = 1 + 0.01 * rng.standard_normal(50)
-    u = 0.02 * rng.standard_normal((50, 3))
-    f = WS.equilibrium(rho, u)
-    c = np.asarray(L.D3Q19.VECTORS, float)
-    np.testing.assert_allclose(f.sum(1), rho, rtol=1e-14)
-    np.testing.assert_allclose(f @ c, rho[:, None] * u, rtol=1e-12, atol=1e-16)
-
-
 def test_run_specs_are_seeded_and_fit_the_crop():
     side = 64
     for regime in ("slow", "fast"):
