@@ -1461,10 +1461,10 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* b
   // changes (both state buffers, the iolet stash) and the time bookkeeping.
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
   const size_t sbytes = storage_bytes(s->p.storage) * s->Q * s->np;
-  void* saved[2] = {nullptr, nullptr};
+  DevBuf<unsigned char> saved[2];  // freed on every return path
   for (int b = 0; b < 2; ++b) {
-    HIP_OR_FAIL(hipMalloc(&saved[b], sbytes));
-    HIP_OR_FAIL(hipMemcpyAsync(saved[b], s->state[b], sbytes, hipMemcpyDeviceToDevice,
+    HIP_OR_FAIL(saved[b].alloc(sbytes));
+    HIP_OR_FAIL(hipMemcpyAsync(saved[b].p, s->state[b], sbytes, hipMemcpyDeviceToDevice,
                                s->stream));
   }
   const size_t ni = std::max<size_t>(1, s->io_sites_internal.size());
@@ -1507,7 +1507,7 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* b
   }
   // restore
   for (int b = 0; b < 2; ++b) {
-    if (hipMemcpyAsync(s->state[b], saved[b], sbytes, hipMemcpyDeviceToDevice, s->stream) !=
+    if (hipMemcpyAsync(s->state[b], saved[b].p, sbytes, hipMemcpyDeviceToDevice, s->stream) !=
         hipSuccess)
       rc = fail(HLB_E_DEVICE, "state restore failed");
   }
@@ -1515,7 +1515,6 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* b
                      s->stream) != hipSuccess)
     rc = fail(HLB_E_DEVICE, "state restore failed");
   (void)hipStreamSynchronize(s->stream);
-  for (int b = 0; b < 2; ++b) (void)hipFree(saved[b]);
   s->cur = cur;
   s->t_ref = t_ref;
   s->post_stream = post_stream;
