@@ -84,7 +84,6 @@ struct StepArgs {
   // Boundary values of the step being streamed: per iolet, two doubles
   // (Nash: ghost density; Ladd Womersley: e^{i w t}; Ladd parabolic: speed).
   const double* bv = nullptr;
-  int* unstable = nullptr;  // set to 1 if a post-stream f is not > 0
 
   // Launch shape of the step kernel in the block layout: threads per
   // workgroup (64, 128 or 256), x planes per wave, workgroups in the grid

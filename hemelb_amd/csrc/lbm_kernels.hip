@@ -765,15 +765,9 @@ __device__ __forceinline__ void step_site(const DevDomain& D, const StepArgs& A,
   using T = typename Store<S>::T;
   T f[Q];
   gather<Q, S, T, Lay, BULK>(D, A, fin, r, f);
-  bool ok = true;
-  sfor<0, Q>([&](auto di) {
-    constexpr int d = decltype(di)::value;
-    if constexpr (kShifted<S>)
-      ok = ok && ((double)f[d] + Lat<Q>::w[d] * A.rho0 > 0.0);
-    else
-      ok = ok && (f[d] > T(0.0));
-  });
-  if (!ok) atomicOr(A.unstable, 1);
+  // Stability (every f > 0) is tested on the state by the monitor's
+  // reduction, as the reference's StabilityTester tests fNew
+  // (StabilityTester.h:97-113), not inside the step.
   StepArgs a = A;
   Lay::relax(D, r, a);
   const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, a);

@@ -248,7 +248,6 @@ struct hlb_solver {
   hipEvent_t bv_done[2] = {nullptr, nullptr};
   int bv_cap = 0;
   int bv_which = 0;
-  DevBuf<int> unstable;
   // scratch for exports and monitors, allocated on first use
   DevBuf<double> s_soa, s_aos, rho, u, u_snap, partials, gathered;
   DevBuf<int64_t> slots;
@@ -868,8 +867,6 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
     HIP_OR_FAIL(s->blk_omega_f.upload(h_bomega_f));
     HIP_OR_FAIL(s->blk_s_tau.upload(h_bstau));
   }
-  HIP_OR_FAIL(s->unstable.alloc(1));
-  HIP_OR_FAIL(hipMemset(s->unstable.p, 0, sizeof(int)));
   s->bv_cap = 512;
   const size_t nio = std::max<size_t>(1, s->iolets.size());
   for (int b = 0; b < 2; ++b) {
@@ -925,7 +922,6 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   A.omega = -1.0 / P.tau;  // LbmParameters::CalcDerivedParams
   A.omega_f = (float)A.omega;
   A.rho0 = P.storage == HLB_STORE_FP64 ? 0.0 : P.rho0;
-  A.unstable = s->unstable.p;
   // MRT relaxation rate of the viscous moments (DHumieres*MRTBasis.cc)
   A.mrt_s_tau = 1.0 / P.tau;
   // Default launch shape: 128-thread workgroups; runs of 8 workgroups per
