@@ -127,6 +127,18 @@ __device__ __forceinline__ T dotc(T a, T b, T c) {
   else return T(0);
 }
 
+// a*b + c.  Binary32 (the reduced-precision forms, which follow no reference
+// rounding) fuses it into one FMA; binary64 keeps the reference's separately
+// rounded product and sum.
+__device__ __forceinline__ float mad(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
+
+// 1/x.  Binary32: the hardware reciprocal (1 ulp), binary64: division.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // ---------------------------------------------------------------------------
 // Storage: binary64 holds f; binary32/binary16 hold f - w_i*rho0.
 // ---------------------------------------------------------------------------
@@ -211,15 +223,15 @@ struct FeqShifted {
   T drho, q, nine, mx, my, mz;
   __device__ __forceinline__ FeqShifted(T drho_, T rho, T mx_, T my_, T mz_)
       : drho(drho_), mx(mx_), my(my_), mz(mz_) {
-    const T inv = T(1) / rho;
-    q = drho - T(1.5) * ((mx * mx + my * my) + mz * mz) * inv;
+    const T inv = recip(rho);
+    q = mad(T(-1.5) * mad(mz, mz, mad(my, my, mx * mx)), inv, drho);
     nine = T(4.5) * inv;
   }
   template <int Q, int I>
   __device__ __forceinline__ T eq() const {
     using L = Lat<Q>;
     const T e = dotc<L::c[I][0], L::c[I][1], L::c[I][2]>(mx, my, mz);
-    return T(L::w[I]) * ((q + nine * (e * e)) + T(3.) * e);
+    return T(L::w[I]) * mad(T(3.), e, mad(nine, e * e, q));
   }
 };
 
@@ -286,7 +298,7 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
       sfor<0, Q>([&](auto di) {
         constexpr int d = decltype(di)::value;
         const T e = dotc<L::c[d][0], L::c[d][1], L::c[d][2]>(sx, sy, sz);
-        fneq[d] = fneq[d] - T(L::w[d]) * (s0 + e);
+        fneq[d] = mad(T(-L::w[d]), s0 + e, fneq[d]);
       });
     }
   } else {
@@ -302,7 +314,7 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
     const T omega = std::is_same<T, double>::value ? T(a.omega) : T(a.omega_f);
     sfor<0, Q>([&](auto di) {
       constexpr int d = decltype(di)::value;
-      f[d] = f[d] + fneq[d] * omega;
+      f[d] = mad(fneq[d], omega, f[d]);
     });
   } else {
     // MRT: m_neq = M f_neq (MRT.h:123-135); f* = f - sum_k (S_k N_kd) m_neq_k
@@ -339,7 +351,7 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
           } else {
             c = s_tau * N;
           }
-          coll += T(c) * mneq[k];
+          coll = mad(T(c), mneq[k], coll);
         }
       });
       f[d] = f[d] - coll;
@@ -695,7 +707,12 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
       const T qd = (T)(double)q[dp - 1];
       if (((wm >> (dp - 1)) & 1u) && !((im >> (dp - 1)) & 1u) && !((wm >> (e - 1)) & 1u) &&
           !(qd < T(0.5)))
-        f[e] = (f[e] + (T(2.0) * qd - T(1)) * opp[e]) / (T(2.0) * qd);
+      {
+        if constexpr (kShifted<S>)
+          f[e] = mad(T(2.0) * qd - T(1), opp[e], f[e]) * recip(T(2.0) * qd);
+        else
+          f[e] = (f[e] + (T(2.0) * qd - T(1)) * opp[e]) / (T(2.0) * qd);
+      }
     });
   }
   if (im) {
