@@ -102,7 +102,7 @@ struct StepArgs {
   // One-pass shapes: run the planes without boundary sites and those with
   // them as two launches of separately compiled kernels (1) or all planes
   // in one launch (0).  The bulk kernel needs far fewer registers (FP64
-  // 86 instead of 116 VGPRs, FP32/FP16 46 instead of 76), but on the
+  // 84 instead of 110 VGPRs, FP32/FP16 45-48 instead of 76-78), but on the
   // vessel tree the two serial launches measured 6-7 % slower for FP64 and
   // FP32 and 1.4 % faster for FP16, so one launch is the default and the
   // split is a tuner candidate.

@@ -61,3 +61,19 @@ def test_physical_fields_use_reference_units():
     assert abs(p.pressure[0] - (1.0 - 1 / 3) * lattice_pressure / 133.3223874) < 1e-9
     assert abs(p.velocity[0, 0] - 0.01 * 0.8) < 1e-15
     assert abs(p.wall_shear_stress[0] - 0.002 * lattice_pressure) < 1e-12
+
+
+def test_traffic_records_name_a_full_launch_shape():
+    """bench.py attaches a profiled traffic figure only when the record's
+    launch shape equals the solver's, field for field."""
+    import json
+    import os
+    from hemelb_amd import _native as N
+    path = os.path.join(os.path.dirname(__file__), "..", "profiles", "traffic.json")
+    recs = json.load(open(path))
+    fields = {name for name, _ in N.hlb_launch_shape._fields_}
+    for key in ("tree5_fp64", "tree5_fp32", "tree5_fp16"):
+        rec = recs[key]
+        assert set(rec["launch"]) == fields, key
+        assert rec["bytes_per_launch"] == rec["fetch_bytes"] + rec["write_bytes"]
+        assert rec["sites"] == 7929603
