@@ -36,17 +36,24 @@ struct Grid3 {
   __host__ __device__ __forceinline__ int64_t P() const { return (int64_t)N * D * H * W; }
 };
 
-// Neighbour of position p at tap t, or -1 outside the volume (zero padding).
-__device__ __forceinline__ int64_t tap_neighbour(const Grid3& g, int64_t p, int t) {
-  const int w = (int)(p % g.W);
-  const int64_t r1 = p / g.W;
-  const int h = (int)(r1 % g.H);
-  const int64_t r2 = r1 / g.H;
-  const int d = (int)(r2 % g.D);
-  const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-  const int dd = d + dz, hh = h + dy, ww = w + dx;
-  if (dd < 0 || dd >= g.D || hh < 0 || hh >= g.H || ww < 0 || ww >= g.W) return -1;
-  return p + ((int64_t)dz * g.H + dy) * g.W + dx;
+// Coordinates (w, h, d) of position p inside its sample (P < 2^31).
+struct Coord3 {
+  int w, h, d;
+};
+__device__ __forceinline__ Coord3 coord_of(const Grid3& g, int p) {
+  const int r1 = p / g.W, r2 = r1 / g.H;
+  return Coord3{p - r1 * g.W, r1 - r2 * g.H, r2 % g.D};
+}
+// The coordinates of position p + step from those of p (no division).
+__device__ __forceinline__ void advance(const Grid3& g, Coord3& c, int step) {
+  c.w += step;
+  while (c.w >= g.W) {
+    c.w -= g.W;
+    if (++c.h == g.H) {
+      c.h = 0;
+      if (++c.d == g.D) c.d = 0;
+    }
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -80,8 +87,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t* __res
 #pragma unroll
   for (int j = 0; j < CT; ++j) acc[j] = float4_t{0.f, 0.f, 0.f, 0.f};
   const int ntile = min(CT, co / 16 - tile0);
+  // the lane's position decomposed once; each tap checks its neighbour
+  // against the volume (zero padding outside)
+  const Coord3 pc = pvalid ? coord_of(g, (int)p) : Coord3{0, 0, 0};
   for (int t = 0; t < 27; ++t) {
-    const int64_t q = pvalid ? tap_neighbour(g, p, t) : -1;
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const bool inside = pvalid && (unsigned)(pc.d + dz) < (unsigned)g.D &&
+                        (unsigned)(pc.h + dy) < (unsigned)g.H &&
+                        (unsigned)(pc.w + dx) < (unsigned)g.W;
+    const int64_t q = inside ? p + ((int64_t)dz * g.H + dy) * g.W + dx : -1;
     for (int c0 = 0; c0 < ci; c0 += 16) {
       short4_t a = short4_t{0, 0, 0, 0};
       if (q >= 0) a = *reinterpret_cast<const short4_t*>(x + q * ci + c0 + 4 * kq);
@@ -147,8 +161,12 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Grid3 g, const uint16_t* __r
 #pragma unroll
   for (int i = 0; i < TG; ++i) acc[i] = float4_t{0.f, 0.f, 0.f, 0.f};
   const int pbeg = k * chunk, pend = min(P, pbeg + chunk);
-  const int HW = g.H * g.W, DHW = g.D * HW;
   const int kd = tg * TG / 9 - 1;  // TG = 9: one kd plane per tap group
+  // Coordinates of the lane's four positions, advanced by 16 per step
+  // instead of divided out again.
+  Coord3 pc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) pc[j] = coord_of(g, min(pbeg + 4 * kq + j, P - 1));
   for (int pk = pbeg; pk < pend; pk += 16) {
     short4_t a = short4_t{0, 0, 0, 0};
     int pos[4];
@@ -158,8 +176,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Grid3 g, const uint16_t* __r
       pos[j] = pk + 4 * kq + j;
       if (pos[j] < pend) {
         a[j] = (short)dy[pos[j] * co + oc];
-        const int d2 = (pos[j] % DHW) / HW + kd;
-        if (d2 >= 0 && d2 < g.D) okd |= 1u << j;
+        if ((unsigned)(pc[j].d + kd) < (unsigned)g.D) okd |= 1u << j;
       }
     }
 #pragma unroll
@@ -170,12 +187,14 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Grid3 g, const uint16_t* __r
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if ((okd >> j) & 1u) {
-          const int h2 = (pos[j] / g.W) % g.H + dyy, w2 = pos[j] % g.W + dx;
-          if (h2 >= 0 && h2 < g.H && w2 >= 0 && w2 < g.W) b[j] = (short)x[(pos[j] + off) * ci + ic];
+          if ((unsigned)(pc[j].h + dyy) < (unsigned)g.H && (unsigned)(pc[j].w + dx) < (unsigned)g.W)
+            b[j] = (short)x[(pos[j] + off) * ci + ic];
         }
       }
       acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tt], 0, 0, 0);
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) advance(g, pc[j], 16);
   }
 #pragma unroll
   for (int tt = 0; tt < TG; ++tt) {
