@@ -205,18 +205,52 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Grid3 g, const uint16_t* __r
   }
 }
 
-// out[i] = sum over k of part[k][i], k in order (deterministic).
-__global__ void k_sum_chunks(const float* __restrict__ part, int64_t nchunks, int64_t n,
-                             float* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// out[i] = sum over k of part[k][i] in a fixed order (deterministic): a
+// 256-thread block covers kOut consecutive outputs; each of its 256 / kOut
+// phases adds the chunks k = phase, phase + phases, ... in order, and the
+// phase sums are then added in phase order.
+template <int kOut>
+__global__ __launch_bounds__(256) void k_sum_chunks(const float* __restrict__ part,
+                                                    int64_t nchunks, int64_t n,
+                                                    float* __restrict__ out) {
+  constexpr int kPhases = 256 / kOut;
+  __shared__ float sh[256];
+  const int j = (int)threadIdx.x % kOut, q = (int)threadIdx.x / kOut;
+  const int64_t i = (int64_t)blockIdx.x * kOut + j;
   float s = 0.f;
-  for (int64_t k = 0; k < nchunks; ++k) s += part[k * n + i];
-  out[i] = s;
+  if (i < n)
+    for (int64_t k = q; k < nchunks; k += kPhases) s += part[k * n + i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (q == 0 && i < n) {
+    float t = sh[j];
+    for (int r = 1; r < kPhases; ++r) t += sh[r * kOut + j];
+    out[i] = t;
+  }
 }
 
-// Per-channel sums of a float [P][C] tensor over a chunk of positions:
-// part[chunk][c]; one thread per (chunk, channel).
+// part[k][c] = sum of a[p][c] over the positions p of chunk k, for C
+// dividing 256: one block per chunk reads the chunk's [positions][C] slab
+// with consecutive threads on consecutive elements (thread t always on
+// channel t % C), then adds the threads of each channel in a fixed tree.
+__global__ __launch_bounds__(256) void k_channel_sum_block(const float* __restrict__ a,
+                                                           int64_t P, int C, int64_t chunk,
+                                                           float* __restrict__ part) {
+  __shared__ float sh[256];
+  const int64_t k = blockIdx.x;
+  const int64_t e0 = k * chunk * C, e1 = min(P, (k + 1) * chunk) * C;
+  float s = 0.f;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) s += a[e];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int stride = 128; stride >= C; stride >>= 1) {
+    if ((int)threadIdx.x < stride) sh[threadIdx.x] += sh[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < C) part[k * C + threadIdx.x] = sh[threadIdx.x];
+}
+
+// Per-channel sums for any C: one thread per (chunk, channel).
 __global__ void k_channel_sum_part(const float* __restrict__ a, int64_t P, int C,
                                    int64_t chunk, float* __restrict__ part) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -421,6 +455,16 @@ __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict
 
 inline dim3 blocks_for(int64_t n, int t = 256) { return dim3((unsigned)((n + t - 1) / t)); }
 
+// out[i] = sum over k of part[k * n + i] (fixed order).
+void sum_chunks(const float* part, int64_t nchunks, int64_t n, float* out, hipStream_t s) {
+  if (n >= 64)
+    hipLaunchKernelGGL(k_sum_chunks<64>, blocks_for(n, 64), dim3(256), 0, s, part, nchunks, n,
+                       out);
+  else
+    hipLaunchKernelGGL(k_sum_chunks<16>, blocks_for(n, 16), dim3(256), 0, s, part, nchunks, n,
+                       out);
+}
+
 }  // namespace
 
 int conv_fwd(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* w,
@@ -463,7 +507,7 @@ int conv_wgrad(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* dy
   hipLaunchKernelGGL((k_conv_wgrad<TG>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g,
                      x, ci, dy, co, chunk, part);
   const int64_t n = (int64_t)27 * co * ci;
-  hipLaunchKernelGGL(k_sum_chunks, blocks_for(n), dim3(256), 0, s, part, nchunks, n, dw);
+  sum_chunks(part, nchunks, n, dw, s);
   return (int)hipGetLastError();
 }
 
@@ -471,9 +515,13 @@ int channel_sum(const float* a, int64_t P, int C, float* part, int64_t chunk, fl
                 void* stream) {
   const int64_t nchunks = (P + chunk - 1) / chunk;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_channel_sum_part, blocks_for(nchunks * C), dim3(256), 0, s, a, P, C,
-                     chunk, part);
-  hipLaunchKernelGGL(k_sum_chunks, blocks_for(C), dim3(256), 0, s, part, nchunks, (int64_t)C, out);
+  if (C > 0 && 256 % C == 0)
+    hipLaunchKernelGGL(k_channel_sum_block, dim3((unsigned)nchunks), dim3(256), 0, s, a, P, C,
+                       chunk, part);
+  else
+    hipLaunchKernelGGL(k_channel_sum_part, blocks_for(nchunks * C), dim3(256), 0, s, a, P, C,
+                       chunk, part);
+  sum_chunks(part, nchunks, (int64_t)C, out, s);
   return (int)hipGetLastError();
 }
 
@@ -554,8 +602,7 @@ int head_bwd(int64_t P, int K, int O, const float* dout, const uint16_t* h, cons
   const int64_t nchunks = (P + chunk - 1) / chunk;
   hipLaunchKernelGGL(k_head_wgrad_part, blocks_for(nchunks * O * K), dim3(256), 0, s, P, K, O,
                      dout, h, chunk, part);
-  hipLaunchKernelGGL(k_sum_chunks, blocks_for((int64_t)O * K), dim3(256), 0, s, part, nchunks,
-                     (int64_t)O * K, dW);
+  sum_chunks(part, nchunks, (int64_t)O * K, dW, s);
   return channel_sum(dout, P, O, part, chunk, db, stream);
 }
 
