@@ -248,7 +248,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip timing the reduced-precision storage forms")
-    ap.add_argument("--form-steps", type=int, default=500)
+    ap.add_argument("--form-steps", type=int, default=0,
+                    help="timed steps of each other storage form (0: as many as --steps)")
     ap.add_argument("--precision-report", action="store_true")
     ap.add_argument("--report-steps", type=int, default=0,
                     help="steps per precision-report run (default: the workload's)")
@@ -330,10 +331,11 @@ def main() -> int:
             other = work.solver(storage=st, layout=args.layout)
             if launch and (args.threads or args.xcd_remap >= 0 or args.split >= 0):
                 other.set_launch_shape(**launch)
-            f_ms, f_mlups = timed_rate(other, n, args.form_steps, args.warmup)
+            f_steps = args.form_steps or args.steps
+            f_ms, f_mlups = timed_rate(other, n, f_steps, args.warmup)
             f_ach = 2 * 19 * STORE_BYTES[st] * n / (f_ms * 1e-3) / 1e9
             forms[st] = {"value": round(f_mlups, 2), "unit": "MLUPS",
-                         "ms_per_step": round(f_ms, 5), "steps": args.form_steps,
+                         "ms_per_step": round(f_ms, 5), "steps": f_steps,
                          "ratio_to_headline": round(f_mlups / mlups, 3),
                          "achieved_GBs": round(f_ach, 1)}
             del other
