@@ -18,6 +18,10 @@
  *    the last failure on the calling thread.
  *  - A solver is used from one host thread at a time; all of its device work
  *    is ordered on its own HIP stream.
+ *  - After HLB_E_DEVICE from a call that advances or overwrites the state
+ *    (hlb_step, hlb_time_steps, the set/import calls), the state and time
+ *    step are unspecified: destroy the solver, or set the distributions and
+ *    time step again before further steps.
  *  - Site arrays describe the reference's geometry::Domain for one rank
  *    (Code/geometry/Domain.h:53-531): n sites in the caller's local order,
  *    with wall/iolet link bitmasks (bit d-1 for lattice direction d,
