@@ -473,7 +473,11 @@ int conv_fwd(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* w,
   const Grid3 g{cg.N, cg.D, cg.H, cg.W};
   const int64_t P = (int64_t)cg.N * cg.D * cg.H * cg.W;
   const int tiles = co / 16;
-  const int tpw = tiles >= 8 ? 8 : tiles;  // tiles per wave
+  int tpw = tiles >= 8 ? 8 : tiles;  // tiles per wave
+  // Coarse levels have few positions: fewer tiles per wave, so that the grid
+  // still has about four waves per SIMD (the input patches are then loaded
+  // once per tile group).
+  while (tpw > 1 && ((P + 15) / 16) * ((tiles + tpw - 1) / tpw) < 4096) tpw = (tpw + 1) / 2;
   const int ngroups = (tiles + tpw - 1) / tpw;
   const int64_t waves = ((P + 15) / 16) * ngroups;
   const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
