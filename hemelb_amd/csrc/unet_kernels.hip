@@ -455,14 +455,10 @@ __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict
 
 inline dim3 blocks_for(int64_t n, int t = 256) { return dim3((unsigned)((n + t - 1) / t)); }
 
-// out[i] = sum over k of part[k * n + i] (fixed order).
+// out[i] = sum over k of part[k * n + i] (fixed order): 16 outputs per
+// block in 16 chunk phases.
 void sum_chunks(const float* part, int64_t nchunks, int64_t n, float* out, hipStream_t s) {
-  if (n >= 64)
-    hipLaunchKernelGGL(k_sum_chunks<64>, blocks_for(n, 64), dim3(256), 0, s, part, nchunks, n,
-                       out);
-  else
-    hipLaunchKernelGGL(k_sum_chunks<16>, blocks_for(n, 16), dim3(256), 0, s, part, nchunks, n,
-                       out);
+  hipLaunchKernelGGL(k_sum_chunks<16>, blocks_for(n, 16), dim3(256), 0, s, part, nchunks, n, out);
 }
 
 }  // namespace
@@ -493,8 +489,10 @@ int conv_fwd(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* w,
 }
 
 int64_t wgrad_chunk(const ConvGeom& cg) {
+  // up to 2048 chunks: with one wave per (chunk, tap plane, 16x16 tile) the
+  // finest level still has thousands of waves
   const int64_t P = (int64_t)cg.N * cg.D * cg.H * cg.W;
-  int64_t chunk = (P + 511) / 512;
+  int64_t chunk = (P + 2047) / 2048;
   chunk = (chunk + 15) / 16 * 16;
   return chunk < 256 ? 256 : chunk;
 }
