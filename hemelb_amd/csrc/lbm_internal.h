@@ -103,8 +103,8 @@ struct StepArgs {
   // them as two launches of separately compiled kernels (1) or all planes
   // in one launch (0).  The bulk kernel needs far fewer registers (FP64
   // 84 instead of 110 VGPRs, FP32/FP16 45-48 instead of 76-78), but on the
-  // vessel tree the two serial launches measured 6-7 % slower for FP64 and
-  // FP32 and 1.4 % faster for FP16, so one launch is the default and the
+  // vessel tree the two serial launches measured slower in every form
+  // (FP64 7 %, FP32 13 %, FP16 1.5 %), so one launch is the default and the
   // split is a tuner candidate.
   uint32_t split = 0;
 };
