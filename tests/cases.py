@@ -221,3 +221,39 @@ def tree_case(generations=3, root_radius=8.0, seed=0, Q=19, kernel="LBGK", wall=
     _, outs = pressure_iolets(dom, 1.0, 0.0, period, 1.0)
     ins = womersley_inlets(dom, 0.05, period, tau)
     return Case(dom, kernel, wall, "LADD", "NASH", tau, ins, outs, rho0=1.0)
+
+
+# ---------------------------------------------------------------------------
+# Oracle runs of the benchmark workloads (hemelb_amd.workloads)
+# ---------------------------------------------------------------------------
+def _oracle_iolet(io) -> O.orc_iolet:
+    if isinstance(io, L.InOutLetCosine):
+        return O.make_iolet(O.ORC_COSINE, io.position, io.normal,
+                            density_mean=io.density_mean, density_amp=io.density_amp,
+                            phase=io.phase, period=io.period, warmup=int(io.warmup))
+    if isinstance(io, L.InOutLetWomersleyVelocity):
+        return O.make_iolet(O.ORC_WOMERSLEY, io.position, io.normal, radius=io.radius,
+                            pressure_gradient_amplitude=io.pressure_gradient_amplitude,
+                            period=io.period, womersley_number=io.womersley_number)
+    if isinstance(io, L.InOutLetParabolicVelocity):
+        return O.make_iolet(O.ORC_PARABOLIC, io.position, io.normal, radius=io.radius,
+                            max_speed=io.max_speed, warmup=int(io.warmup))
+    raise TypeError(type(io))
+
+
+_ORC_BC = {L.NashZerothOrderPressure: O.ORC_IOLET_NASH, L.LaddIolet: O.ORC_IOLET_LADD}
+_ORC_WALL = {L.SimpleBounceBack: O.ORC_WALL_SBB, L.BouzidiFirdaousLallemand: O.ORC_WALL_BFL}
+_ORC_KERNEL = {L.LBGK: O.ORC_LBGK, L.MRT: O.ORC_MRT}
+
+
+def workload_oracle(w) -> O.OracleSim:
+    """The oracle set up like ``w.solver()``: the same domain, traits,
+    relaxation time and iolets, rest equilibrium at w.rho0."""
+    d, tr = w.domain, w.traits
+    sim = O.OracleSim(d.Q, _ORC_KERNEL[tr.Kernel], _ORC_WALL[tr.WallStreamer],
+                      _ORC_BC[tr.InletStreamer], _ORC_BC[tr.OutletStreamer], w.tau,
+                      d.coords, d.site_type, d.wall_mask, d.iolet_mask, d.iolet_id, d.cut,
+                      [_oracle_iolet(i) for i in w.inlets],
+                      [_oracle_iolet(o) for o in w.outlets])
+    sim.set_equilibrium(w.rho0)
+    return sim

@@ -569,3 +569,200 @@ def test_threaded_step_equals_single_thread():
     finally:
         O.lib().orc_set_threads(n0)
     np.testing.assert_array_equal(a.get_f(), b.get_f())
+
+
+# ---------------------------------------------------------------------------
+# The iolet time index inside a step
+# ---------------------------------------------------------------------------
+# SimulationState starts at t = 1 (SimulationState.cc:15-16).  During the
+# stream of step t a Nash link takes the iolet density at the 0-indexed step
+# t - 1 (BoundaryValues.cc:162-165), a Ladd link the iolet velocity at the
+# 1-indexed t (LaddIolet.h:52, BoundaryValues.h:67-70).  Each test below
+# checks every iolet slot of one step against a textbook formula evaluated
+# at that index, and that the neighbouring indices would miss by far more
+# than the tolerance, so an off-by-one in either direction fails.
+def _perturbed(dom, Q, seed):
+    c, w, _ = _lattice(Q)
+    rng = np.random.default_rng(seed)
+    return w * (1.0 + 0.02 * rng.normal(size=(dom.n, Q))) + w * 0.01 * (c @ [0.0, 0.0, 1.0])
+
+
+def _nash_expected(case, f0, rho_of_site):
+    """Slot values of every Nash link after one step from f0: the ghost
+    equilibrium (textbook form) at the iolet density, with the momentum's
+    projection on the float-rounded normal."""
+    dom, Q = case.domain, case.domain.Q
+    c, w, inv = _lattice(Q)
+    out = {}
+    for i in np.nonzero(dom.iolet_mask)[0]:
+        spec = case.inlets[0] if dom.site_type[i] == 2 else case.outlets[0]
+        n = np.asarray(spec.normal, float)
+        nf = (n / np.linalg.norm(n)).astype(np.float32).astype(float)
+        rho_g = rho_of_site(i)
+        m = c.T @ f0[i]
+        mg = nf * ((m @ nf) / f0[i].sum()) * rho_g
+        feq = _textbook_feq(Q, rho_g, mg)
+        for dp in range(1, Q):
+            if (dom.iolet_mask[i] >> (dp - 1)) & 1:
+                out[(i, inv[dp])] = feq[inv[dp]]
+    return out
+
+
+@pytest.mark.parametrize("t0", [None, 2, 5, 8, 13])
+def test_nash_link_uses_density_of_step_t_minus_1(t0):
+    """A short-period, large-amplitude cosine inlet (T = 8 steps): the Nash
+    slots after the stream of step t equal the ghost equilibrium at
+    rho(t - 1).  t0 = None checks a fresh run, which starts at t = 1 and so
+    streams with rho(0) = mean + amplitude."""
+    Q = 15
+    dom = K.Geometry.cylinder(5.0, 12).project(Q)
+    T, mean, amp, phase = 8.0, 1.02, 0.01, 0.4
+    ins = [K.IoletSpec("cosine", tuple(p.position), tuple(p.normal),
+                       dict(density_mean=mean, density_amp=amp, period=T, phase=phase))
+           for p in dom.inlets]
+    outs = [K.IoletSpec("cosine", tuple(p.position), tuple(p.normal),
+                        dict(density_mean=1.0, density_amp=0.0, period=T))
+            for p in dom.outlets]
+    case = K.Case(dom, "LBGK", "SBB", "NASH", "NASH", 0.8, ins, outs, rho0=1.0)
+    sim = case.oracle()
+    f0 = _perturbed(dom, Q, 11)
+    sim.set_f(f0)
+    if t0 is None:
+        assert sim.time() == 1
+        t = 1
+    else:
+        sim.set_time(t0)
+        t = t0
+    sim.step(1)
+    assert sim.time() == t + 1
+    f1 = sim.get_f()
+
+    def rho_at(tt):
+        def rho(i):
+            if dom.site_type[i] != 2:
+                return 1.0
+            return mean + amp * math.cos(2.0 * math.pi / T * tt + phase)
+        return rho
+
+    want = _nash_expected(case, f0, rho_at(t - 1))
+    assert sum(1 for (i, _) in want if dom.site_type[i] == 2) > 20
+    for (i, e), v in want.items():
+        assert abs(f1[i, e] - v) <= 1e-12, (i, e, f1[i, e], v)
+    # the neighbouring time indices give values far outside the tolerance
+    for other in (t, t - 2):
+        alt = _nash_expected(case, f0, rho_at(other))
+        gaps = [abs(alt[k] - want[k]) for k in want if dom.site_type[k[0]] == 2]
+        assert min(gaps) > 1e-7, (other, min(gaps))
+
+
+def _ladd_expected(case, f0, u_of):
+    """Slot values of every Ladd link after one step from f0:
+    f*_d - 2 w_d rho (u_w . c_d) / Cs2 with u_w at the link's midpoint."""
+    dom, Q = case.domain, case.domain.Q
+    c, w, inv = _lattice(Q)
+    out = {}
+    for i in np.nonzero(dom.iolet_mask)[0]:
+        if dom.site_type[i] != 2:
+            continue
+        fp = _collide_textbook(f0[i], case.tau, Q)
+        rho = f0[i].sum()
+        for dp in range(1, Q):
+            if not (dom.iolet_mask[i] >> (dp - 1)) & 1:
+                continue
+            u = u_of(dom.coords[i] + 0.5 * c[dp])
+            out[(i, inv[dp])] = fp[dp] - 2 * w[dp] * rho * (u @ c[dp]) * 3.0
+    return out
+
+
+def _ladd_case(inlet):
+    Q = 19
+    dom = K.Geometry.cylinder(5.0, 12).project(Q)
+    _, outs = K.pressure_iolets(dom, 1.0, 0.0, 40.0, 1.0)
+    return K.Case(dom, "LBGK", "SBB", "LADD", "NASH", 0.9, [inlet(dom.inlets[0])], outs,
+                  rho0=1.0)
+
+
+@pytest.mark.parametrize("t0", [None, 25, 99])
+def test_ladd_link_uses_one_indexed_step_during_warmup(t0):
+    """Parabolic inlet inside its warm-up (100 steps): the profile scales
+    with t / warmup at the 1-indexed step t being streamed."""
+    umax, warm = 0.02, 100
+
+    def inlet(p):
+        return K.IoletSpec("parabolic", tuple(p.position), tuple(p.normal),
+                           dict(radius=p.radius, max_speed=umax, warmup=warm))
+    case = _ladd_case(inlet)
+    dom = case.domain
+    spec = case.inlets[0]
+    pos, n = np.asarray(spec.position, float), np.asarray(spec.normal, float)
+    sim = case.oracle()
+    f0 = _perturbed(dom, 19, 12)
+    sim.set_f(f0)
+    t = 1 if t0 is None else t0
+    if t0 is not None:
+        sim.set_time(t0)
+    sim.step(1)
+    f1 = sim.get_f()
+
+    def u_at(tt):
+        def u(x):
+            d = x - pos
+            r2 = d @ d - (d @ n) ** 2
+            return n * umax * min(1.0, tt / warm) * (1 - r2 / spec.params["radius"] ** 2)
+        return u
+
+    want = _ladd_expected(case, f0, u_at(t))
+    assert len(want) > 50
+    for (i, e), v in want.items():
+        assert abs(f1[i, e] - v) <= 1e-13, (i, e, f1[i, e], v)
+    for other in (t - 1, t + 1):
+        alt = _ladd_expected(case, f0, u_at(other))
+        assert max(abs(alt[k] - want[k]) for k in want) > 1e-7
+
+
+@pytest.mark.parametrize("t0", [None, 7, 30])
+def test_ladd_link_uses_one_indexed_step_womersley(t0):
+    """Womersley inlet with a 40-step period: the velocity at the link's
+    midpoint is the textbook solution (scipy's complex Bessel function) at
+    the 1-indexed step t.  The reference's J0 series carries ~1e-6 relative
+    error (Bessel.h:33), which sets the tolerance; one step of phase moves
+    the slots by ~1e-4."""
+    from scipy.special import jv
+    T, G, alpha = 40.0, 2e-4, 3.0
+
+    def inlet(p):
+        return K.IoletSpec("womersley", tuple(p.position), tuple(p.normal),
+                           dict(radius=p.radius, pressure_gradient_amplitude=G, period=T,
+                                womersley_number=alpha))
+    case = _ladd_case(inlet)
+    dom = case.domain
+    spec = case.inlets[0]
+    R = spec.params["radius"]
+    pos, n = np.asarray(spec.position, float), np.asarray(spec.normal, float)
+    sim = case.oracle()
+    f0 = _perturbed(dom, 19, 13)
+    sim.set_f(f0)
+    t = 1 if t0 is None else t0
+    if t0 is not None:
+        sim.set_time(t0)
+    sim.step(1)
+    f1 = sim.get_f()
+    w_ = 2 * math.pi / T
+    i32 = complex(math.cos(3 * math.pi / 4), math.sin(3 * math.pi / 4))
+
+    def u_at(tt):
+        def u(x):
+            d = x - pos
+            r = math.sqrt(max(0.0, d @ d - (d @ n) ** 2))
+            a = G / w_ * (1 - jv(0, i32 * alpha * r / R) / jv(0, i32 * alpha))
+            return n * -(a * complex(math.cos(w_ * tt), math.sin(w_ * tt))).real
+        return u
+
+    want = _ladd_expected(case, f0, u_at(t))
+    assert len(want) > 50
+    scale = 2 * (1 / 18) * 3 * G / w_
+    for (i, e), v in want.items():
+        assert abs(f1[i, e] - v) <= 1e-5 * scale, (i, e, f1[i, e], v)
+    for other in (t - 1, t + 1):
+        alt = _ladd_expected(case, f0, u_at(other))
+        assert max(abs(alt[k] - want[k]) for k in want) > 1e-2 * scale
