@@ -100,6 +100,7 @@ _SIGS = {
     "hlb_get_wall_shear_stress": (C.c_int, [_P, _P]),
     "hlb_get_stress": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "hlb_reduce": (C.c_int, [_P, C.POINTER(hlb_monitor)]),
+    "hlb_reduce_noalloc": (C.c_int, [_P, C.POINTER(hlb_monitor)]),
     "hlb_create_ensemble": (C.c_int, [C.POINTER(hlb_params), C.POINTER(hlb_domain), _P,
                                       C.c_int32, _P, C.POINTER(hlb_iolet), C.c_int32,
                                       C.POINTER(hlb_iolet), C.c_int32, C.POINTER(_P)]),

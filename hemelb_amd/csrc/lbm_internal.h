@@ -144,6 +144,25 @@ int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
 int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
                   const double* rho, const double* u, double* partials,
                   int nblocks, void* stream);
+// Monitor statistics of the reference's fOld read straight from the state
+// `fin` (post-stream when from_post_stream, else pulled as
+// launch_stream_only does): partials[5*g..] = mass, min rho, max rho,
+// max |u|, unstable of workgroup g < kMonitorGroups, and io_vals[4*r..] =
+// rho, u of iolet-site record r < d.ni.
+constexpr int kMonitorGroups = 1024;
+int launch_monitor(const DevDomain& d, const StepArgs& a, const void* fin,
+                   int from_post_stream, double* partials, double* io_vals, void* stream);
+
+// The library's static device scratch (a file-scope __device__ array of
+// kStaticScratchDoubles binary64 values, 8 MiB, one copy per GPU held until
+// the process ends): the monitor partials followed by the values of up to
+// kStaticIoletSites iolet sites, for the allocation-free monitor.
+// scratch_address gives its address on `device`, which must be the current
+// device on the first call for that device (hipGetSymbolAddress, cached per
+// device); returns 0 or a HIP error code.
+constexpr int64_t kStaticIoletSites = int64_t(1) << 18;
+constexpr int64_t kStaticScratchDoubles = 5 * kMonitorGroups + 4 * kStaticIoletSites;
+int scratch_address(int device, double** out);
 // Stress tensor [6][np] (xx xy xz yy yz zz), von Mises [np] and shear rate
 // [np] of every site of a binary64 SoA state (any output may be null); the
 // tau of site x is tau_site[x] when given, else tau.

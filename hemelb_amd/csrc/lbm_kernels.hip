@@ -18,9 +18,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "lbm_internal.h"
+
+// Static scratch of the allocation-free monitor: kMonitorGroups * 5
+// workgroup partials, then (rho, u) of up to kStaticIoletSites iolet sites.
+// Host code never names this array (its name is only a host-side stand-in);
+// it uses the per-device address from hlb::scratch_address.
+static __device__ __align__(256) double scratch[hlb::kStaticScratchDoubles];
+
 
 namespace hlb {
 namespace {
@@ -403,12 +411,17 @@ __device__ __forceinline__ uint32_t step_workgroup(uint32_t remap) {
 // own slot of the inverse direction (x + (inv(e) - e) * np) for a cut link.
 struct SparseLayout {
   static __device__ __forceinline__ bool locate(const DevDomain& D, SiteRef& r) {
-    r.x = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
-    if (r.x >= D.n) return false;
-    const uint32_t bi = D.binfo[r.x];
+    const int64_t x = (int64_t)logical_workgroup() * blockDim.x + threadIdx.x;
+    if (x >= D.n) return false;
+    at(D, x, r);
+    return true;
+  }
+  // Compact site x (x < D.n).
+  static __device__ __forceinline__ void at(const DevDomain& D, int64_t x, SiteRef& r) {
+    r.x = x;
+    const uint32_t bi = D.binfo[x];
     r.cut = bi != 0 ? 1u : 0u;  // only "is boundary" is needed here
     r.rec = bi - 1;
-    return true;
   }
   template <int Q, int E>
   static __device__ __forceinline__ int64_t offset(const DevDomain& D, const SiteRef& r) {
@@ -1021,6 +1034,95 @@ __global__ __launch_bounds__(256) void k_reduce(int64_t n, int64_t np,
   }
 }
 
+// The monitor's statistics straight from the state, without an exported
+// copy: every site's fOld (the state itself when it is post-stream, else the
+// pull of k_stream_only), its density and velocity as k_moments computes
+// them, and per workgroup [mass, min rho, max rho, max |u|, unstable] as
+// k_reduce combines them.  Workgroup g visits the x planes g*W + k*W*grid
+// (W waves per workgroup) of the block layouts, or the compact sites
+// g*256 + t + k*256*grid of the pull table, so the partials depend on the
+// grid size only.  The density and velocity of every iolet site go to
+// io_vals[4*record..] (rho, u_x, u_y, u_z) for the iolet fluxes.
+template <int Q, typename S, class Lay>
+__global__ __launch_bounds__(256) void k_monitor(DevDomain D, StepArgs A,
+                                                 const S* __restrict__ fin,
+                                                 int from_post_stream,
+                                                 double* __restrict__ partials,
+                                                 double* __restrict__ io_vals) {
+  using T = typename Store<S>::T;
+  __shared__ double s_sum[256], s_min[256], s_max[256], s_u[256], s_bad[256];
+  const int t = threadIdx.x;
+  double sum = 0.0, mn = 1.7976931348623157e308, mx = -1.7976931348623157e308,
+         um = 0.0, bad = 0.0;
+  auto visit = [&](const SiteRef& r) {
+    T f[Q];
+    if (from_post_stream) {
+      sfor<0, Q>([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        f[d] = Store<S>::ld(fin + (int64_t)d * D.np + r.x);
+      });
+    } else {
+      gather<Q, S, T, Lay>(D, A, fin, r, f);
+    }
+    double g[Q];
+    sfor<0, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      double v = (double)f[d];
+      if constexpr (kShifted<S>) v += Lat<Q>::w[d] * A.rho0;
+      g[d] = v;
+      if (!(v > 0.0)) bad = 1.0;
+    });
+    double rho, m0, m1, m2;
+    moments<Q, double>(g, rho, m0, m1, m2);
+    const double ux = m0 / rho, uy = m1 / rho, uz = m2 / rho;
+    sum += rho;
+    mn = rho < mn ? rho : mn;
+    mx = rho > mx ? rho : mx;
+    const double m = sqrt(((0.0 + ux * ux) + uy * uy) + uz * uz);
+    um = m > um ? m : um;
+    if (r.cut) {
+      const int io = D.b_io[r.rec];
+      if (io >= 0) {
+        io_vals[4 * (int64_t)io] = rho;
+        io_vals[4 * (int64_t)io + 1] = ux;
+        io_vals[4 * (int64_t)io + 2] = uy;
+        io_vals[4 * (int64_t)io + 3] = uz;
+      }
+    }
+  };
+  if constexpr (kWaveLayout<Lay>) {
+    const uint32_t wpg = blockDim.x >> 6;
+    for (uint32_t base = blockIdx.x * wpg; base < D.nwaves; base += gridDim.x * wpg) {
+      const uint32_t w = __builtin_amdgcn_readfirstlane(base + ((uint32_t)t >> 6));
+      SiteRef r;
+      if (Lay::locate_wave(D, w, r)) visit(r);
+    }
+  } else {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + t; x < D.n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+      SiteRef r;
+      Lay::at(D, x, r);
+      visit(r);
+    }
+  }
+  s_sum[t] = sum; s_min[t] = mn; s_max[t] = mx; s_u[t] = um; s_bad[t] = bad;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s_sum[t] += s_sum[t + w];
+      s_min[t] = s_min[t + w] < s_min[t] ? s_min[t + w] : s_min[t];
+      s_max[t] = s_max[t + w] > s_max[t] ? s_max[t + w] : s_max[t];
+      s_u[t] = s_u[t + w] > s_u[t] ? s_u[t + w] : s_u[t];
+      s_bad[t] = s_bad[t + w] > s_bad[t] ? s_bad[t + w] : s_bad[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* p = partials + 5 * blockIdx.x;
+    p[0] = s_sum[0]; p[1] = s_min[0]; p[2] = s_max[0]; p[3] = s_u[0]; p[4] = s_bad[0];
+  }
+}
+
 // The same statistics per chunk of sites [range[2b], range[2b+1]) (one
 // workgroup per chunk; chunks never mix instances of an ensemble).
 template <int Q>
@@ -1504,6 +1606,35 @@ int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
     hipLaunchKernelGGL(k_reduce<19>, dim3(nblocks), dim3(256), 0,
                        (hipStream_t)stream, n, np, soa, rho, u, partials);
   return (int)hipGetLastError();
+}
+
+int scratch_address(int device, double** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    // the caller has made `device` current: this is its copy of the array
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    addr[device] = (double*)p;
+  }
+  *out = addr[device];
+  return 0;
+}
+
+int launch_monitor(const DevDomain& d, const StepArgs& a, const void* fin,
+                   int from_post_stream, double* partials, double* io_vals, void* stream) {
+  return dispatch(d, a, [&](auto q, auto, auto* s, auto* lay) {
+    using S = std::remove_pointer_t<decltype(s)>;
+    using Lay = std::remove_pointer_t<decltype(lay)>;
+    hipLaunchKernelGGL((k_monitor<decltype(q)::value, S, Lay>), dim3(kMonitorGroups),
+                       dim3(256), 0, (hipStream_t)stream, d, a, (const S*)fin,
+                       from_post_stream, partials, io_vals);
+    return (int)hipGetLastError();
+  });
 }
 
 int launch_stress_fields(int Q, int64_t n, int64_t np, const double* soa, double tau,

@@ -249,7 +249,9 @@ struct hlb_solver {
   int bv_cap = 0;
   int bv_which = 0;
   // scratch for exports and monitors, allocated on first use
-  DevBuf<double> s_soa, s_aos, rho, u, u_snap, partials, gathered;
+  DevBuf<double> s_soa, s_aos, rho, u, u_snap, partials, gathered, mon_io;
+  // this device's copy of the library's static scratch (hlb_reduce_noalloc)
+  double* scratch = nullptr;
   DevBuf<int64_t> slots;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -356,11 +358,12 @@ struct hlb_solver {
     hlb::StepArgs a = args;
     // the pending stream belongs to reference step t_ref - 1
     if (!post_stream && !iolets.empty()) {
-      std::vector<double> v(2 * iolets.size());
-      boundary_values(t_ref - 1, v.data());
-      // stage through the first bv buffer (synchronously; exports are rare)
+      // stage through the first pinned and device bv buffers, synchronously
+      // (exports and monitors are rare); once the stream is idle no step's
+      // copy still reads the pinned buffer
       (void)hipStreamSynchronize(stream);
-      (void)hipMemcpy(bv[0].p, v.data(), sizeof(double) * v.size(),
+      boundary_values(t_ref - 1, bv_host[0]);
+      (void)hipMemcpy(bv[0].p, bv_host[0], sizeof(double) * 2 * iolets.size(),
                       hipMemcpyHostToDevice);
       a.bv = bv[0].p;
     }
@@ -822,6 +825,8 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
 
   // Device set-up.
   HIP_OR_FAIL(hipSetDevice(P.device));
+  if (hlb::scratch_address(P.device, &s->scratch))
+    return fail(HLB_E_DEVICE, "static scratch of the device not found");
   HIP_OR_FAIL(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   HIP_OR_FAIL(hipEventCreate(&s->ev0));
   HIP_OR_FAIL(hipEventCreate(&s->ev1));
@@ -1230,49 +1235,24 @@ int hlb_get_stress(hlb_solver* s, double* tensor, double* von_mises, double* she
   return HLB_OK;
 }
 
-int hlb_reduce(hlb_solver* s, hlb_monitor* out) {
-  if (!s || !out) return fail(HLB_E_INVALID, "null argument");
-  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
-  int r = compute_moments(s);
-  if (r) return r;
-  const int nb = 1024;
-  if (!s->partials.p) HIP_OR_FAIL(s->partials.alloc(5 * nb));
-  if (hlb::launch_reduce(s->Q, s->n, s->npc, s->s_soa.p, s->rho.p, s->u.p, s->partials.p, nb,
-                         s->stream))
-    return fail(HLB_E_DEVICE, "reduction kernel launch failed");
-  std::vector<double> part(5 * nb);
-  HIP_OR_FAIL(hipMemcpyAsync(part.data(), s->partials.p, sizeof(double) * 5 * nb,
-                             hipMemcpyDeviceToHost, s->stream));
-  // Iolet fluxes: gather rho and u at the iolet sites.
-  const int64_t ni = (int64_t)s->io_sites_internal.size();
-  std::vector<double> g(4 * std::max<int64_t>(ni, 1));
-  if (ni) {
-    std::vector<int64_t> slots(4 * ni);
-    for (int64_t k = 0; k < ni; ++k) {
-      const int64_t o = s->io_sites_internal[k];
-      slots[k] = o;                         // rho
-      slots[ni + k] = o;                    // u_x
-      slots[2 * ni + k] = s->npc + o;       // u_y
-      slots[3 * ni + k] = 2 * s->npc + o;   // u_z
-    }
-    DevBuf<int64_t> d_slots;
-    DevBuf<double> d_g;
-    HIP_OR_FAIL(d_slots.upload(slots));
-    HIP_OR_FAIL(d_g.alloc(4 * ni));
-    if (hlb::launch_gather_pairs(ni, s->npc, d_slots.p, s->rho.p, 0, 0.0, nullptr, d_g.p, s->stream) ||
-        hlb::launch_gather_pairs(3 * ni, s->npc, d_slots.p + ni, s->u.p, 0, 0.0, nullptr,
-                                 d_g.p + ni, s->stream))
-      return fail(HLB_E_DEVICE, "gather kernel launch failed");
-    HIP_OR_FAIL(hipMemcpyAsync(g.data(), d_g.p, sizeof(double) * 4 * ni, hipMemcpyDeviceToHost,
-                               s->stream));
-    HIP_OR_FAIL(hipStreamSynchronize(s->stream));
-  }
-  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
-  hlb_monitor m{};
+// The monitor's statistics (hlb_reduce, hlb_reduce_noalloc): one launch of
+// k_monitor over the state into `partials` and `io_vals`.
+static int launch_monitor_into(hlb_solver* s, double* partials, double* io_vals) {
+  const hlb::StepArgs a = s->args_for_stream();
+  if (hlb::launch_monitor(s->dom, a, s->state[s->cur], s->post_stream ? 1 : 0, partials,
+                          io_vals, s->stream))
+    return fail(HLB_E_DEVICE, "monitor kernel launch failed");
+  return HLB_OK;
+}
+
+// Workgroup partials [mass, min rho, max rho, max |u|, unstable] combined
+// in workgroup order.
+static void combine_partials(const hlb_solver* s, const double* part, hlb_monitor& m) {
+  m = hlb_monitor{};
   m.density_min = 1.7976931348623157e308;
   m.density_max = -1.7976931348623157e308;
   bool bad = false;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = 0; b < hlb::kMonitorGroups; ++b) {
     const double* p = &part[5 * b];
     m.mass += p[0];
     m.density_min = std::min(m.density_min, p[1]);
@@ -1282,11 +1262,71 @@ int hlb_reduce(hlb_solver* s, hlb_monitor* out) {
   }
   m.stable = bad ? 0 : 1;
   m.n_iolets = (int32_t)s->iolets.size();
-  for (int64_t k = 0; k < ni; ++k) {
-    const int io = s->io_sites_iolet[k];
+}
+
+// Iolet fluxes: sum over the iolet sites (records k0 .. k0+count-1, in
+// record order) of rho (u . n); vals[4*(k-k0)..] = rho, u of record k.
+static void add_fluxes(const hlb_solver* s, const double* vals, int64_t k0, int64_t count,
+                       hlb_monitor& m) {
+  for (int64_t j = 0; j < count; ++j) {
+    const int io = s->io_sites_iolet[k0 + j];
     const double* nrm = s->iolets[io].cfg.normal;
-    const double un = ((0.0 + g[ni + k] * nrm[0]) + g[2 * ni + k] * nrm[1]) + g[3 * ni + k] * nrm[2];
-    if (io < 64) m.iolet_flux[io] += g[k] * un;
+    const double* v = vals + 4 * j;
+    const double un = ((0.0 + v[1] * nrm[0]) + v[2] * nrm[1]) + v[3] * nrm[2];
+    if (io < 64) m.iolet_flux[io] += v[0] * un;
+  }
+}
+
+int hlb_reduce(hlb_solver* s, hlb_monitor* out) {
+  if (!s || !out) return fail(HLB_E_INVALID, "null argument");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  const int nb = hlb::kMonitorGroups;
+  const int64_t ni = (int64_t)s->io_sites_internal.size();
+  if (!s->partials.p) HIP_OR_FAIL(s->partials.alloc(5 * nb));
+  if (!s->mon_io.p) HIP_OR_FAIL(s->mon_io.alloc(4 * std::max<int64_t>(ni, 1)));
+  int r = launch_monitor_into(s, s->partials.p, s->mon_io.p);
+  if (r) return r;
+  std::vector<double> part(5 * nb), g(4 * std::max<int64_t>(ni, 1));
+  HIP_OR_FAIL(hipMemcpyAsync(part.data(), s->partials.p, sizeof(double) * 5 * nb,
+                             hipMemcpyDeviceToHost, s->stream));
+  if (ni)
+    HIP_OR_FAIL(hipMemcpyAsync(g.data(), s->mon_io.p, sizeof(double) * 4 * ni,
+                               hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  hlb_monitor m;
+  combine_partials(s, part.data(), m);
+  add_fluxes(s, g.data(), 0, ni, m);
+  *out = m;
+  return HLB_OK;
+}
+
+int hlb_reduce_noalloc(hlb_solver* s, hlb_monitor* out) {
+  if (!s || !out) return fail(HLB_E_INVALID, "null argument");
+  if (!s->initialised) return fail(HLB_E_STATE, "state not initialised");
+  const int64_t ni = (int64_t)s->io_sites_internal.size();
+  if (ni > hlb::kStaticIoletSites)
+    return fail(HLB_E_INVALID, "hlb_reduce_noalloc: " + std::to_string(ni) +
+                                   " iolet sites exceed the static scratch limit of " +
+                                   std::to_string(hlb::kStaticIoletSites) +
+                                   "; use hlb_reduce");
+  double* partials = s->scratch;
+  double* io_vals = s->scratch + 5 * hlb::kMonitorGroups;
+  int r = launch_monitor_into(s, partials, io_vals);
+  if (r) return r;
+  // host copies into fixed buffers on the stack: no allocation anywhere
+  double part[5 * hlb::kMonitorGroups];
+  HIP_OR_FAIL(hipMemcpyAsync(part, partials, sizeof(part), hipMemcpyDeviceToHost, s->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  hlb_monitor m;
+  combine_partials(s, part, m);
+  constexpr int64_t kChunk = 512;
+  double vals[4 * kChunk];
+  for (int64_t k0 = 0; k0 < ni; k0 += kChunk) {
+    const int64_t cnt = std::min<int64_t>(kChunk, ni - k0);
+    HIP_OR_FAIL(hipMemcpyAsync(vals, io_vals + 4 * k0, sizeof(double) * 4 * cnt,
+                               hipMemcpyDeviceToHost, s->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+    add_fluxes(s, vals, k0, cnt, m);
   }
   *out = m;
   return HLB_OK;

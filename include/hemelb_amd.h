@@ -208,6 +208,18 @@ int hlb_get_stress(hlb_solver* s, double* tensor, double* von_mises, double* she
 
 int hlb_reduce(hlb_solver* s, hlb_monitor* out);
 
+/* hlb_reduce without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernel as hlb_reduce fills, instead of solver-owned buffers, the library's
+ * static device scratch (8 MiB per GPU, held until the process ends), so the
+ * results are identical to hlb_reduce's.  Limit: at most 262144 iolet sites
+ * (HLB_E_INVALID beyond).  Every solver on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (two host threads, or two solvers whose
+ * streams run concurrently) race; the call synchronises the solver's stream
+ * before it returns. */
+int hlb_reduce_noalloc(hlb_solver* s, hlb_monitor* out);
+
 /* ---- Batched ensemble --------------------------------------------------
  * No counterpart in the reference, which runs one vessel per job.  Several
  * independent vessels share one solver: instance[i] in [0, n_instances)
