@@ -8,7 +8,7 @@ from . import _native  # noqa: F401
 from .config import SimBuilder, SimConfig, read_xml, write_xml  # noqa: F401
 from .geometry import Domain, Geometry, IoletPlane  # noqa: F401
 from .lb import (D3Q15, D3Q19, LBGK, LBM, MRT, BouzidiFirdaousLallemand,  # noqa: F401
-                 InOutLetCosine, InOutLetParabolicVelocity, InOutLetWomersleyVelocity,
+                 HaloPlan, InOutLetCosine, InOutLetParabolicVelocity, InOutLetWomersleyVelocity,
                  LaddIolet, LbmParameters, NashZerothOrderPressure, SimpleBounceBack,
                  Traits, UnitConverter)
 
@@ -17,5 +17,5 @@ __all__ = [
     "SimpleBounceBack", "BouzidiFirdaousLallemand", "NashZerothOrderPressure",
     "LaddIolet", "InOutLetCosine", "InOutLetParabolicVelocity",
     "InOutLetWomersleyVelocity", "LbmParameters", "Traits", "UnitConverter",
-    "SimConfig", "SimBuilder", "read_xml", "write_xml",
+    "SimConfig", "SimBuilder", "read_xml", "write_xml", "HaloPlan",
 ]

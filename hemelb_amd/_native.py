@@ -67,6 +67,12 @@ class hlb_launch_shape(C.Structure):
                 ("xcd_remap", C.c_int32), ("split", C.c_int32)]
 
 
+class hlb_copy_result(C.Structure):
+    _fields_ = [("bytes_per_s", C.c_double), ("ms", C.c_double), ("bytes", C.c_int64),
+                ("threads", C.c_int32), ("xcd_remap", C.c_int32), ("width", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 HLBX_FLOAT, HLBX_DOUBLE, HLBX_INT32, HLBX_UINT32, HLBX_INT64, HLBX_UINT64 = range(6)
 
 
@@ -101,6 +107,7 @@ _SIGS = {
     "hlb_get_stress": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "hlb_reduce": (C.c_int, [_P, C.POINTER(hlb_monitor)]),
     "hlb_reduce_noalloc": (C.c_int, [_P, C.POINTER(hlb_monitor)]),
+    "hlb_copy_ceiling": (C.c_int, [_P, C.c_int32, C.POINTER(hlb_copy_result)]),
     "hlb_create_ensemble": (C.c_int, [C.POINTER(hlb_params), C.POINTER(hlb_domain), _P,
                                       C.c_int32, _P, C.POINTER(hlb_iolet), C.c_int32,
                                       C.POINTER(hlb_iolet), C.c_int32, C.POINTER(_P)]),
@@ -115,6 +122,16 @@ _SIGS = {
     "hlb_snapshot_velocity": (C.c_int, [_P]),
     "hlb_velocity_change": (C.c_int, [_P, C.c_double, C.POINTER(C.c_double)]),
     "hlb_halo_pack": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "hlb_create_subdomain": (C.c_int, [C.POINTER(hlb_params), C.POINTER(hlb_domain),
+                                       C.POINTER(hlb_domain), C.POINTER(hlb_iolet), C.c_int32,
+                                       C.POINTER(hlb_iolet), C.c_int32, C.POINTER(_P)]),
+    "hlb_halo_plan_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "hlb_halo_plan_destroy": (None, [_P]),
+    "hlb_halo_plan_count": (C.c_int64, [_P]),
+    "hlb_halo_exchange": (C.c_int, [_P]),
+    "hlb_halo_pack_device": (C.c_int, [_P, _P]),
+    "hlb_halo_unpack_device": (C.c_int, [_P, _P]),
+    "hlb_halo_last_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hlb_time_steps": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_double)]),
     "hlb_site_count": (C.c_int64, [_P]),
     "hlb_lattice_q": (C.c_int32, [_P]),

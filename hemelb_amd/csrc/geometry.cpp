@@ -1006,10 +1006,12 @@ int hlbg_read_gmy(const char* path, hlbg_geometry** out) {
       const int bx = (int)(b / ((int64_t)g->blocks[1] * g->blocks[2]));
       const int by = (int)((b / g->blocks[2]) % g->blocks[1]);
       const int bz = (int)(b % g->blocks[2]);
+      uint32_t fluid = 0;
       for (int l = 0; l < bs * bs * bs; ++l) {  // ParseSite (GeometryReader.cc:556-653)
         uint32_t type = in.u32();
         if (type == 0) continue;
         if (type != 1) return fail("invalid site type");
+        ++fluid;
         const int lx = l / (bs * bs), ly = (l / bs) % bs, lz = l % bs;
         g->coords.push_back(bx * bs + lx);
         g->coords.push_back(by * bs + ly);
@@ -1044,6 +1046,14 @@ int hlbg_read_gmy(const char* path, hlbg_geometry** out) {
           g->link_index.push_back(-1);
         }
       }
+      // the header declares each block's fluid-site count and uncompressed
+      // length; a block that disagrees is malformed
+      if (fluid != nfl[b])
+        return fail("block " + std::to_string(b) + " holds " + std::to_string(fluid) +
+                    " fluid sites, its header declares " + std::to_string(nfl[b]));
+      if (in.i != (size_t)ulen[b])
+        return fail("block " + std::to_string(b) + " has " +
+                    std::to_string((size_t)ulen[b] - in.i) + " unread bytes");
     }
     *out = g.release();
     return 0;

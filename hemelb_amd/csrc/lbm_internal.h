@@ -144,6 +144,18 @@ int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
 int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
                   const double* rho, const double* u, double* partials,
                   int nblocks, void* stream);
+// Device-copy ceiling: Q arrays of `bytes` bytes (array d at byte offset
+// d * stride_bytes) copied from `in` to `out` in words of `width` bytes
+// (4, 8 or 16), one word per thread and array, with the step's workgroup
+// size and workgroup-to-XCD remap.
+int launch_copy_arrays(int Q, int64_t stride_bytes, int64_t bytes, int width,
+                       const void* in, void* out, uint32_t threads, uint32_t remap,
+                       void* stream);
+// Halo exchange of a subdomain pair: pack (pack != 0) copies
+// from[idx[k]] to to[k], unpack copies from[k] to to[idx[k]], for k < count,
+// as raw stored words of esize bytes (8, 4 or 2).
+int launch_halo(int64_t count, const int64_t* idx, int esize, const void* from, void* to,
+                int pack, void* stream);
 // Monitor statistics of the reference's fOld read straight from the state
 // `fin` (post-stream when from_post_stream, else pulled as
 // launch_stream_only does): partials[5*g..] = mass, min rho, max rho,

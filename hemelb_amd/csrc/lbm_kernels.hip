@@ -29,7 +29,6 @@
 // it uses the per-device address from hlb::scratch_address.
 static __device__ __align__(256) double scratch[hlb::kStaticScratchDoubles];
 
-
 namespace hlb {
 namespace {
 
@@ -917,6 +916,46 @@ __global__ __launch_bounds__(256) void k_stream_only(DevDomain D, StepArgs A,
   });
 }
 
+// Device-copy ceiling of the step's traffic: the Q state arrays (stride
+// `stride` words of type W) copied from `in` to `out`, `words` per array,
+// one word per thread and array, all loads issued before the streamed
+// stores, workgroups mapped to XCDs as the step maps them.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int Q, typename W>
+__global__ __launch_bounds__(256) void k_copy_arrays(int64_t stride, int64_t words,
+                                                     const W* __restrict__ in,
+                                                     W* __restrict__ out, uint32_t remap) {
+  const int64_t i = (int64_t)step_workgroup(remap) * blockDim.x + threadIdx.x;
+  if (i >= words) return;
+  W v[Q];
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    v[d] = __builtin_nontemporal_load(in + (int64_t)d * stride + i);
+  });
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    __builtin_nontemporal_store(v[d], out + (int64_t)d * stride + i);
+  });
+}
+
+// Halo exchange between subdomains: stored values copied as raw words
+// (E = the storage's 8-, 4- or 2-byte word), from state to a buffer (pack)
+// or from a buffer to state (unpack), at precomputed state indices.
+template <typename E>
+__global__ void k_halo_pack(int64_t count, const int64_t* __restrict__ idx,
+                            const E* __restrict__ state, E* __restrict__ buf) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) buf[k] = state[idx[k]];
+}
+template <typename E>
+__global__ void k_halo_unpack(int64_t count, const int64_t* __restrict__ idx,
+                              const E* __restrict__ buf, E* __restrict__ state) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) state[idx[k]] = buf[k];
+}
+
 // Caller-order AoS [n][Q] -> state, perm[compact] = caller index.
 template <int Q, typename S, class Lay>
 __global__ __launch_bounds__(256) void k_import(DevDomain D, const double* aos,
@@ -1605,6 +1644,58 @@ int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
   else
     hipLaunchKernelGGL(k_reduce<19>, dim3(nblocks), dim3(256), 0,
                        (hipStream_t)stream, n, np, soa, rho, u, partials);
+  return (int)hipGetLastError();
+}
+
+int launch_copy_arrays(int Q, int64_t stride_bytes, int64_t bytes, int width,
+                       const void* in, void* out, uint32_t threads, uint32_t remap,
+                       void* stream) {
+  if (width != 4 && width != 8 && width != 16) return (int)hipErrorInvalidValue;
+  if (stride_bytes % width || bytes % width) return (int)hipErrorInvalidValue;
+  if (threads != 64 && threads != 128 && threads != 256) return (int)hipErrorInvalidValue;
+  const int64_t words = bytes / width, stride = stride_bytes / width;
+  uint64_t groups = ((uint64_t)words + threads - 1) / threads;
+  if (remap >= 2) {
+    const uint64_t m = 8ull * remap;
+    groups = (groups + m - 1) / m * m;
+  }
+  const dim3 grid((unsigned)std::max<uint64_t>(groups, 1)), block(threads);
+  auto go = [&](auto q, auto* w) {
+    using W = std::remove_pointer_t<decltype(w)>;
+    hipLaunchKernelGGL((k_copy_arrays<decltype(q)::value, W>), grid, block, 0,
+                       (hipStream_t)stream, stride, words, (const W*)in, (W*)out, remap);
+  };
+  if (Q == 15) {
+    if (width == 4) go(std::integral_constant<int, 15>{}, (unsigned int*)nullptr);
+    else if (width == 8) go(std::integral_constant<int, 15>{}, (u32x2*)nullptr);
+    else go(std::integral_constant<int, 15>{}, (u32x4*)nullptr);
+  } else if (Q == 19) {
+    if (width == 4) go(std::integral_constant<int, 19>{}, (unsigned int*)nullptr);
+    else if (width == 8) go(std::integral_constant<int, 19>{}, (u32x2*)nullptr);
+    else go(std::integral_constant<int, 19>{}, (u32x4*)nullptr);
+  } else {
+    return (int)hipErrorInvalidValue;
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_halo(int64_t count, const int64_t* idx, int esize, const void* from, void* to,
+                int pack, void* stream) {
+  if (count == 0) return 0;
+  const dim3 grid = grid_for(count), block(256);
+  auto go = [&](auto* e) {
+    using E = std::remove_pointer_t<decltype(e)>;
+    if (pack)
+      hipLaunchKernelGGL(k_halo_pack<E>, grid, block, 0, (hipStream_t)stream, count, idx,
+                         (const E*)from, (E*)to);
+    else
+      hipLaunchKernelGGL(k_halo_unpack<E>, grid, block, 0, (hipStream_t)stream, count, idx,
+                         (const E*)from, (E*)to);
+  };
+  if (esize == 8) go((uint64_t*)nullptr);
+  else if (esize == 4) go((uint32_t*)nullptr);
+  else if (esize == 2) go((uint16_t*)nullptr);
+  else return (int)hipErrorInvalidValue;
   return (int)hipGetLastError();
 }
 

@@ -196,6 +196,12 @@ uint64_t spread3(uint64_t v) {
   return x;
 }
 
+// Absolute coordinate key of a site (21 bits per axis, offset 2^20).
+inline uint64_t coord_key(const int32_t* c) {
+  return ((uint64_t)((int64_t)c[0] + (1 << 20)) << 42) |
+         ((uint64_t)((int64_t)c[1] + (1 << 20)) << 21) | (uint64_t)((int64_t)c[2] + (1 << 20));
+}
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -276,6 +282,13 @@ struct hlb_solver {
   DevBuf<int64_t> d_wall_sites;
   DevBuf<float> d_wall_normals;
   DevBuf<double> d_wss;
+  // subdomains (hlb_create_subdomain): ghost sites in internal order n..,
+  // their cut-link masks and coordinates, and every owned or ghost site by
+  // absolute coordinate key (sorted) for the halo plans
+  bool subdomain = false;
+  std::vector<uint32_t> ghost_cut;
+  std::vector<int32_t> ghost_coords;
+  std::vector<std::pair<uint64_t, int64_t>> site_by_key;
   std::vector<int64_t> io_sites_internal;  // per iolet-site record
   std::vector<int32_t> io_sites_iolet;
   uint64_t t_ref = 1;
@@ -398,8 +411,12 @@ double hlb_pressure_gradient_to_lattice(double pg, double dt, double dx, double 
 }
 
 // Builds a solver; `instance` (optional, [n]) assigns every site to one of
-// n_inst independent vessels relaxed with inst_tau[k] (an ensemble).
-static int create_impl(const hlb_params* params, const hlb_domain* dm,
+// n_inst independent vessels relaxed with inst_tau[k] (an ensemble); `gh`
+// (optional) lists ghost sites: sites of neighbouring subdomains whose
+// post-collision values this subdomain's sites pull across the cut.  Ghost
+// sites get state slots that the pull reads and the halo exchange fills;
+// the step never updates them.
+static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb_domain* gh,
                        const int32_t* instance, int32_t n_inst, const double* inst_tau,
                        const hlb_iolet* inlets, int32_t n_inlets,
                        const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
@@ -441,6 +458,16 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
     for (int64_t i = 0; i < dm->n_sites; ++i)
       if (instance[i] < 0 || instance[i] >= n_inst)
         return fail(HLB_E_INVALID, "site " + std::to_string(i) + ": instance out of range");
+  }
+
+  const int64_t ng = gh ? gh->n_sites : 0;
+  if (gh) {
+    if (ng < 0 || ng >= (int64_t)0x7FFFFFF0ll - n) return fail(HLB_E_INVALID, "ghost count out of range");
+    if (ng && (!gh->coords || !gh->wall_mask || !gh->iolet_mask))
+      return fail(HLB_E_INVALID, "ghost arrays missing");
+    if (instance) return fail(HLB_E_INVALID, "ensembles take no ghost sites");
+    if (P.layout == HLB_LAYOUT_COMPACT_BLOCKS)
+      return fail(HLB_E_INVALID, "ghost sites need the block or pull-table layout");
   }
 
   auto s = std::make_unique<hlb_solver>();
@@ -520,6 +547,8 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
   for (int a = 0; a < 3; ++a) lo[a] = dm->coords[a];
   for (int64_t i = 0; i < n; ++i)
     for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], dm->coords[3 * i + a]);
+  for (int64_t g = 0; g < ng; ++g)
+    for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], gh->coords[3 * g + a]);
   auto key_of = [&](int64_t i) {
     const uint64_t x = (uint64_t)(dm->coords[3 * i] - lo[0]),
                    y = (uint64_t)(dm->coords[3 * i + 1] - lo[1]),
@@ -546,6 +575,26 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
     if (s->has_normals)
       for (int a = 0; a < 3; ++a) s->wall_normals.push_back(dm->wall_normal[3 * i + a]);
   }
+  // Ghost sites in Morton order, internal indices n .. n+ng-1.
+  std::vector<std::pair<uint64_t, int64_t>> gkeys(ng);
+  for (int64_t g = 0; g < ng; ++g) {
+    const uint64_t x = (uint64_t)(gh->coords[3 * g] - lo[0]),
+                   y = (uint64_t)(gh->coords[3 * g + 1] - lo[1]),
+                   z = (uint64_t)(gh->coords[3 * g + 2] - lo[2]);
+    const uint64_t oct = (spread3(x >> 3) << 2) | (spread3(y >> 3) << 1) | spread3(z >> 3);
+    gkeys[g] = {(oct << 9) | (((x & 7) << 6) | ((y & 7) << 3) | (z & 7)), g};
+  }
+  std::sort(gkeys.begin(), gkeys.end());
+  for (int64_t g = 1; g < ng; ++g)
+    if (gkeys[g].first == gkeys[g - 1].first)
+      return fail(HLB_E_INVALID, "two ghost sites share the same coordinates");
+  s->ghost_cut.resize(ng);
+  s->ghost_coords.resize(3 * ng);
+  for (int64_t g = 0; g < ng; ++g) {
+    const int64_t i = gkeys[g].second;
+    s->ghost_cut[g] = gh->wall_mask[i] | gh->iolet_mask[i];
+    for (int a = 0; a < 3; ++a) s->ghost_coords[3 * g + a] = gh->coords[3 * i + a];
+  }
   // Site lookup by coordinates: block key -> internal index of slot 0..511.
   std::unordered_map<uint64_t, std::vector<int32_t>> blocks;
   blocks.reserve(n / 64 + 16);
@@ -560,6 +609,15 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
     auto& b = blocks[bkey(x >> 3, y >> 3, z >> 3)];
     if (b.empty()) b.assign(512, -1);
     b[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)] = (int32_t)o;
+  }
+  for (int64_t g = 0; g < ng; ++g) {
+    const int64_t x = s->ghost_coords[3 * g] - lo[0], y = s->ghost_coords[3 * g + 1] - lo[1],
+                  z = s->ghost_coords[3 * g + 2] - lo[2];
+    auto& b = blocks[bkey(x >> 3, y >> 3, z >> 3)];
+    if (b.empty()) b.assign(512, -1);
+    int32_t& slot = b[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)];
+    if (slot >= 0) return fail(HLB_E_INVALID, "a ghost site is also a site of the domain");
+    slot = (int32_t)(n + g);
   }
   auto lookup = [&](int64_t x, int64_t y, int64_t z) -> int64_t {
     if (x < 0 || y < 0 || z < 0) return -1;
@@ -577,11 +635,37 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
     blk_of[o] = (int64_t)blk_keys.size() - 1;
   }
   const int64_t nblocks = (int64_t)blk_keys.size();
-  s->np = compact_store ? s->npc : nblocks * 512;
+  // ghost sites outside the domain's blocks live in ghost blocks after them
+  std::vector<int64_t> gblk_of(ng);
+  std::vector<uint64_t> gblk_keys;
+  for (int64_t g = 0; g < ng; ++g) {
+    const uint64_t k = gkeys[g].first >> 9;
+    auto it = std::lower_bound(blk_keys.begin(), blk_keys.end(), k);
+    if (it != blk_keys.end() && *it == k) {
+      gblk_of[g] = it - blk_keys.begin();
+    } else {
+      if (gblk_keys.empty() || gblk_keys.back() != k) gblk_keys.push_back(k);
+      gblk_of[g] = nblocks + (int64_t)gblk_keys.size() - 1;
+    }
+  }
+  const int64_t nblocks_all = nblocks + (int64_t)gblk_keys.size();
+  s->np = compact_store ? (n + ng + 255) / 256 * 256 : nblocks_all * 512;
   const int64_t np = s->np;
-  s->slot_of.resize(n);
+  s->slot_of.resize(n + ng);
   for (int64_t o = 0; o < n; ++o)
     s->slot_of[o] = compact_store ? o : ((blk_of[o] << 9) | (int64_t)(keys[o].first & 511));
+  for (int64_t g = 0; g < ng; ++g)
+    s->slot_of[n + g] =
+        compact_store ? n + g : ((gblk_of[g] << 9) | (int64_t)(gkeys[g].first & 511));
+  if (gh) {
+    s->subdomain = true;
+    s->site_by_key.reserve(n + ng);
+    for (int64_t o = 0; o < n; ++o)
+      s->site_by_key.push_back({coord_key(&dm->coords[3 * s->perm[o]]), o});
+    for (int64_t g = 0; g < ng; ++g)
+      s->site_by_key.push_back({coord_key(&s->ghost_coords[3 * g]), n + g});
+    std::sort(s->site_by_key.begin(), s->site_by_key.end());
+  }
 
   // Pull table, boundary and iolet records.
   std::vector<int32_t> h_src;
@@ -608,7 +692,9 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
       const int64_t j = lookup(x - L.c[e][0], y - L.c[e][1], z - L.c[e][2]);
       // The source pushes into this slot unless its link e is cut.
       bool pushes = false;
-      if (j >= 0) {
+      if (j >= n) {
+        pushes = !((s->ghost_cut[j - n] >> (e - 1)) & 1u);
+      } else if (j >= 0) {
         const int64_t ci = s->perm[j];
         pushes = !(((dm->wall_mask[ci] | dm->iolet_mask[ci]) >> (e - 1)) & 1u);
         if (pushes && instance && instance[ci] != instance[i])
@@ -701,6 +787,11 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
     }
     for (int64_t b = 0; b < nblocks; ++b)
       ordinal[bkey(bxyz[3 * b], bxyz[3 * b + 1], bxyz[3 * b + 2])] = (int32_t)b;
+    for (int64_t g = 0; g < ng; ++g)
+      if (gblk_of[g] >= nblocks)
+        ordinal[bkey((s->ghost_coords[3 * g] - lo[0]) >> 3,
+                     (s->ghost_coords[3 * g + 1] - lo[1]) >> 3,
+                     (s->ghost_coords[3 * g + 2] - lo[2]) >> 3)] = (int32_t)gblk_of[g];
     h_bnbr.assign((size_t)nblocks * 27, -1);
     for (int64_t b = 0; b < nblocks; ++b)
       for (int dx = -1; dx <= 1; ++dx)
@@ -943,7 +1034,15 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm,
 int hlb_create(const hlb_params* params, const hlb_domain* domain,
                const hlb_iolet* inlets, int32_t n_inlets,
                const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
-  return create_impl(params, domain, nullptr, 0, nullptr, inlets, n_inlets, outlets,
+  return create_impl(params, domain, nullptr, nullptr, 0, nullptr, inlets, n_inlets, outlets,
+                     n_outlets, out);
+}
+
+int hlb_create_subdomain(const hlb_params* params, const hlb_domain* domain,
+                         const hlb_domain* ghosts, const hlb_iolet* inlets, int32_t n_inlets,
+                         const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
+  if (!ghosts) return fail(HLB_E_INVALID, "null ghost domain");
+  return create_impl(params, domain, ghosts, nullptr, 0, nullptr, inlets, n_inlets, outlets,
                      n_outlets, out);
 }
 
@@ -952,8 +1051,8 @@ int hlb_create_ensemble(const hlb_params* params, const hlb_domain* domain,
                         const hlb_iolet* inlets, int32_t n_inlets,
                         const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out) {
   if (!instance) return fail(HLB_E_INVALID, "null instance array");
-  return create_impl(params, domain, instance, n_instances, tau, inlets, n_inlets, outlets,
-                     n_outlets, out);
+  return create_impl(params, domain, nullptr, instance, n_instances, tau, inlets, n_inlets,
+                     outlets, n_outlets, out);
 }
 
 void hlb_destroy(hlb_solver* s) { delete s; }
@@ -1564,6 +1663,48 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* b
   return HLB_OK;
 }
 
+int hlb_copy_ceiling(hlb_solver* s, int32_t reps, hlb_copy_result* out) {
+  if (!s || !out) return fail(HLB_E_INVALID, "null argument");
+  if (reps < 1) return fail(HLB_E_INVALID, "reps must be positive");
+  const int64_t esize = (int64_t)storage_bytes(s->p.storage);
+  const int64_t bytes = s->np * esize;
+  // the other state buffer is free between steps (the next step overwrites it)
+  const void* src = s->state[s->cur];
+  void* dst = s->state[s->cur ^ 1];
+  hlb_copy_result best{};
+  for (int width : {(int)std::max<int64_t>(esize, 4), 16}) {
+    if (bytes % width) continue;
+    for (uint32_t threads : {128u, 256u}) {
+      for (uint32_t remap : {0u, 8u}) {
+        auto launch = [&]() {
+          return hlb::launch_copy_arrays(s->Q, bytes, bytes, width, src, dst, threads, remap,
+                                         s->stream);
+        };
+        if (launch()) return fail(HLB_E_DEVICE, "copy kernel launch failed");
+        HIP_OR_FAIL(hipEventRecord(s->ev0, s->stream));
+        for (int r = 0; r < reps; ++r)
+          if (launch()) return fail(HLB_E_DEVICE, "copy kernel launch failed");
+        HIP_OR_FAIL(hipEventRecord(s->ev1, s->stream));
+        HIP_OR_FAIL(hipEventSynchronize(s->ev1));
+        float ms = 0;
+        HIP_OR_FAIL(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        const double per = (double)ms / reps;
+        const double rate = 2.0 * s->Q * (double)bytes / (per * 1e-3);
+        if (rate > best.bytes_per_s) {
+          best.bytes_per_s = rate;
+          best.ms = per;
+          best.bytes = 2 * (int64_t)s->Q * bytes;
+          best.threads = (int32_t)threads;
+          best.xcd_remap = (int32_t)remap;
+          best.width = width;
+        }
+      }
+    }
+  }
+  *out = best;
+  return HLB_OK;
+}
+
 int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count, double* host_buffer) {
   if (!s || (count && (!pairs || !host_buffer))) return fail(HLB_E_INVALID, "null argument");
   if (s->post_stream)
@@ -1616,6 +1757,154 @@ int hlb_halo_unpack(hlb_solver* s, const int64_t* pairs, int64_t count,
                                 s->weights.p, s->state[s->cur], s->stream))
     return fail(HLB_E_DEVICE, "halo unpack kernel launch failed");
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
+  return HLB_OK;
+}
+
+// ---- Halo plans between subdomains ---------------------------------------
+// The values a subdomain's sites pull across the cut are the neighbouring
+// subdomain's post-collision values of its edge sites: the pairs (site x,
+// direction e) with x + c_e in the other subdomain (the reference's shared
+// distributions, Domain.cc:166-179 and 247-284, sent by
+// FieldData::SendAndReceive and placed by CopyReceived, FieldData.cc:27-49).
+// A plan holds their state indices on both sides, computed once; every
+// exchange is two kernels and an event, with no allocation and no host
+// synchronisation.
+}  // extern "C"
+
+struct hlb_halo_plan {
+  hlb_solver* src = nullptr;
+  hlb_solver* dst = nullptr;
+  int64_t count = 0;
+  int esize = 8;
+  DevBuf<int64_t> src_idx, dst_idx;
+  DevBuf<unsigned char> buf;
+  hipEvent_t packed = nullptr, ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool timed = false;
+  ~hlb_halo_plan() {
+    if (packed) (void)hipEventDestroy(packed);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+extern "C" {
+
+static int64_t find_site(const hlb_solver* s, uint64_t key) {
+  auto it = std::lower_bound(s->site_by_key.begin(), s->site_by_key.end(),
+                             std::make_pair(key, (int64_t)-1));
+  return (it != s->site_by_key.end() && it->first == key) ? it->second : -1;
+}
+
+int hlb_halo_plan_create(hlb_solver* src, hlb_solver* dst, hlb_halo_plan** out) {
+  if (!src || !dst || !out) return fail(HLB_E_INVALID, "null argument");
+  *out = nullptr;
+  if (!src->subdomain || !dst->subdomain)
+    return fail(HLB_E_INVALID, "halo plans join solvers made by hlb_create_subdomain");
+  if (src == dst) return fail(HLB_E_INVALID, "a halo plan joins two different subdomains");
+  if (src->Q != dst->Q || src->p.storage != dst->p.storage || src->p.device != dst->p.device)
+    return fail(HLB_E_INVALID, "subdomains differ in lattice, storage or device");
+  if (src->p.storage != HLB_STORE_FP64 && src->p.rho0 != dst->p.rho0)
+    return fail(HLB_E_INVALID, "subdomains differ in the reduced-precision shift rho0");
+  LatticeHost L(dst->Q);
+  std::vector<std::pair<int64_t, int64_t>> pairs;
+  const int64_t ng = (int64_t)dst->ghost_cut.size();
+  for (int64_t g = 0; g < ng; ++g) {
+    const int32_t* c = &dst->ghost_coords[3 * g];
+    const int64_t o_src = find_site(src, coord_key(c));
+    if (o_src < 0 || o_src >= src->n) continue;  // a ghost of another subdomain
+    for (int e = 1; e < dst->Q; ++e) {
+      if ((dst->ghost_cut[g] >> (e - 1)) & 1u) continue;  // cut link: nothing streams
+      const int32_t t[3] = {c[0] + L.c[e][0], c[1] + L.c[e][1], c[2] + L.c[e][2]};
+      const int64_t o_dst = find_site(dst, coord_key(t));
+      if (o_dst < 0 || o_dst >= dst->n) continue;
+      pairs.push_back({e * src->np + src->slot_of[o_src],
+                       e * dst->np + dst->slot_of[dst->n + g]});
+    }
+  }
+  std::sort(pairs.begin(), pairs.end());
+  auto plan = std::make_unique<hlb_halo_plan>();
+  plan->src = src;
+  plan->dst = dst;
+  plan->count = (int64_t)pairs.size();
+  plan->esize = (int)storage_bytes(src->p.storage);
+  std::vector<int64_t> si(pairs.size()), di(pairs.size());
+  for (size_t k = 0; k < pairs.size(); ++k) {
+    si[k] = pairs[k].first;
+    di[k] = pairs[k].second;
+  }
+  HIP_OR_FAIL(hipSetDevice(src->p.device));
+  HIP_OR_FAIL(plan->src_idx.upload(si));
+  HIP_OR_FAIL(plan->dst_idx.upload(di));
+  HIP_OR_FAIL(plan->buf.alloc((size_t)plan->count * plan->esize));
+  HIP_OR_FAIL(hipEventCreateWithFlags(&plan->packed, hipEventDisableTiming));
+  for (auto& e : plan->ev) HIP_OR_FAIL(hipEventCreate(&e));
+  *out = plan.release();
+  return HLB_OK;
+}
+
+void hlb_halo_plan_destroy(hlb_halo_plan* plan) { delete plan; }
+
+int64_t hlb_halo_plan_count(const hlb_halo_plan* plan) { return plan ? plan->count : -1; }
+
+static int halo_check(const hlb_halo_plan* plan) {
+  if (!plan) return fail(HLB_E_INVALID, "null plan");
+  if (!plan->src->initialised || !plan->dst->initialised)
+    return fail(HLB_E_STATE, "state not initialised");
+  if (plan->src->post_stream || plan->dst->post_stream)
+    return fail(HLB_E_STATE, "no post-collision state yet (step at least once)");
+  return HLB_OK;
+}
+
+int hlb_halo_pack_device(hlb_halo_plan* plan, void* buffer) {
+  int r = halo_check(plan);
+  if (r) return r;
+  if (plan->count && !buffer) return fail(HLB_E_INVALID, "null buffer");
+  hlb_solver* s = plan->src;
+  if (hlb::launch_halo(plan->count, plan->src_idx.p, plan->esize, s->state[s->cur], buffer, 1,
+                       s->stream))
+    return fail(HLB_E_DEVICE, "halo pack kernel launch failed");
+  return HLB_OK;
+}
+
+int hlb_halo_unpack_device(hlb_halo_plan* plan, const void* buffer) {
+  int r = halo_check(plan);
+  if (r) return r;
+  if (plan->count && !buffer) return fail(HLB_E_INVALID, "null buffer");
+  hlb_solver* d = plan->dst;
+  if (hlb::launch_halo(plan->count, plan->dst_idx.p, plan->esize, buffer, d->state[d->cur], 0,
+                       d->stream))
+    return fail(HLB_E_DEVICE, "halo unpack kernel launch failed");
+  return HLB_OK;
+}
+
+int hlb_halo_exchange(hlb_halo_plan* plan) {
+  int r = halo_check(plan);
+  if (r) return r;
+  hlb_solver* s = plan->src;
+  hlb_solver* d = plan->dst;
+  HIP_OR_FAIL(hipEventRecord(plan->ev[0], s->stream));
+  r = hlb_halo_pack_device(plan, plan->buf.p);
+  if (r) return r;
+  HIP_OR_FAIL(hipEventRecord(plan->ev[1], s->stream));
+  HIP_OR_FAIL(hipEventRecord(plan->packed, s->stream));
+  HIP_OR_FAIL(hipStreamWaitEvent(d->stream, plan->packed, 0));
+  HIP_OR_FAIL(hipEventRecord(plan->ev[2], d->stream));
+  r = hlb_halo_unpack_device(plan, plan->buf.p);
+  if (r) return r;
+  HIP_OR_FAIL(hipEventRecord(plan->ev[3], d->stream));
+  plan->timed = true;
+  return HLB_OK;
+}
+
+int hlb_halo_last_times(hlb_halo_plan* plan, double* pack_ms, double* unpack_ms) {
+  if (!plan || !pack_ms || !unpack_ms) return fail(HLB_E_INVALID, "null argument");
+  if (!plan->timed) return fail(HLB_E_STATE, "no exchange yet");
+  HIP_OR_FAIL(hipEventSynchronize(plan->ev[3]));
+  float a = 0, b = 0;
+  HIP_OR_FAIL(hipEventElapsedTime(&a, plan->ev[0], plan->ev[1]));
+  HIP_OR_FAIL(hipEventElapsedTime(&b, plan->ev[2], plan->ev[3]));
+  *pack_ms = a;
+  *unpack_ms = b;
   return HLB_OK;
 }
 

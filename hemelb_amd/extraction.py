@@ -219,12 +219,17 @@ def load_checkpoint(lbm: LBM, path: str, time_step: int | None = None) -> int:
 
 
 def property_writer(lbm: LBM, path: str, units: UnitConverter, origin_m=(0.0, 0.0, 0.0),
-                    reference_pressure_mmHg: float = 0.0) -> ExtractionWriter:
+                    reference_pressure_mmHg: float | None = None) -> ExtractionWriter:
     """Pressure (mmHg, stored as float minus the reference pressure), velocity
     (m/s) and wall shear stress (Pa) as float fields, the reference's default
-    property output types (SimConfig.cc:457)."""
+    property output types (SimConfig.cc:457).  The pressure field's offset is
+    the reference pressure, by default the one `units` converts with (both
+    come from the fluid's reference pressure in the reference: SimConfig.cc:465
+    and UnitConverter.cc:33), so that stored value + offset is the pressure
+    write_properties converted."""
+    ref = units.ref if reference_pressure_mmHg is None else float(reference_pressure_mmHg)
     return ExtractionWriter(path, units.dx, origin_m, lbm.domain.coords, [
-        OutputField("pressure", "float", 1, (reference_pressure_mmHg,)),
+        OutputField("pressure", "float", 1, (ref,)),
         OutputField("velocity", "float", 3),
         OutputField("shearstress", "float", 1)])
 

@@ -177,3 +177,65 @@ class Geometry:
 # Moore neighbourhood in .gmy order (Code/io/formats/geometry.h:120-156)
 MOORE = np.array([(x, y, z) for x in (-1, 0, 1) for y in (-1, 0, 1) for z in (-1, 0, 1)
                   if (x, y, z) != (0, 0, 0)], np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Decomposition into subdomains with ghost sites
+# ---------------------------------------------------------------------------
+def _morton_block_key(coords: np.ndarray) -> np.ndarray:
+    """Morton key of each site's 8^3 block (x, y, z interleaved, x highest)."""
+    b = (coords - coords.min(0)).astype(np.uint64) >> np.uint64(3)
+    key = np.zeros(len(coords), np.uint64)
+    for bit in range(21):
+        for a in range(3):
+            key |= ((b[:, a] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + 2 - a)
+    return key
+
+
+def _subset(d: Domain, sel: np.ndarray) -> Domain:
+    return Domain(d.Q, d.coords[sel], d.site_type[sel], d.wall_mask[sel], d.iolet_mask[sel],
+                  d.iolet_id[sel], d.cut[sel], d.normal[sel], d.inlets, d.outlets)
+
+
+def split_domain(d: Domain, parts: int = 2):
+    """Cuts `d` into `parts` subdomains of contiguous 8^3 blocks in Morton
+    order with about equal site counts (the reference balances ParMETIS
+    partitions; whole blocks keep each subdomain's layout dense).  Returns
+    per part (owned sites, their indices into d, ghost sites): the sites of
+    the other parts from which a site of this part pulls along a fluid link
+    (the reference's shared distributions, Domain.cc:166-179)."""
+    from .lb import LATTICES
+    c = LATTICES[d.Q].VECTORS
+    bkey = _morton_block_key(d.coords)
+    order = np.argsort(bkey, kind="stable")
+    ub, first = np.unique(bkey[order], return_index=True)
+    cum = np.append(first, d.n)
+    part_of_block = np.minimum(
+        (np.searchsorted(cum, (np.arange(1, parts) * d.n) / parts)[None, :] <=
+         np.arange(len(ub))[:, None]).sum(1), parts - 1)
+    part = np.empty(d.n, np.int64)
+    part[order] = part_of_block[np.searchsorted(ub, bkey[order])]
+    # site lookup by coordinate key
+    lo = d.coords.min(0) - 1
+    span = d.coords.max(0) - lo + 2
+    ck = lambda x: ((x[:, 0] - lo[0]) * span[1] + (x[:, 1] - lo[1])) * span[2] + (x[:, 2] - lo[2])  # noqa: E731
+    keys = ck(d.coords.astype(np.int64))
+    ko = np.argsort(keys)
+    ks = keys[ko]
+    cut = d.wall_mask | d.iolet_mask
+    ghost = [np.zeros(d.n, bool) for _ in range(parts)]
+    for e in range(1, d.Q):
+        t = d.coords.astype(np.int64) + c[e]
+        tk = ck(t)
+        pos = np.clip(np.searchsorted(ks, tk), 0, d.n - 1)
+        hit = (ks[pos] == tk) & (((cut >> np.uint32(e - 1)) & 1) == 0)
+        src = np.flatnonzero(hit)
+        dst = ko[pos[hit]]
+        across = part[src] != part[dst]
+        for p in range(parts):
+            ghost[p][src[across & (part[dst] == p)]] = True
+    out = []
+    for p in range(parts):
+        idx = np.flatnonzero(part == p)
+        out.append((_subset(d, idx), idx, _subset(d, np.flatnonzero(ghost[p]))))
+    return out

@@ -255,9 +255,11 @@ class LBM:
     def __init__(self, traits: Traits, domain: Domain, params: LbmParameters,
                  inlets=(), outlets=(), storage: str = "fp64", rho0: float = 1.0,
                  device: int = 0, layout: str = "blocks", instances=None,
-                 instance_tau=None):
+                 instance_tau=None, ghosts: Domain | None = None):
         """`instances` ([n] int, optional) makes an ensemble: independent
-        vessels in one solver, instance k relaxed with instance_tau[k]."""
+        vessels in one solver, instance k relaxed with instance_tau[k].
+        `ghosts` (a Domain of the neighbouring subdomains' sites this one
+        pulls from) makes a subdomain whose ghost slots a HaloPlan fills."""
         if domain.Q != traits.Lattice.NUMVECTORS:
             raise ValueError("domain was projected for a different lattice")
         self.traits, self.domain, self.params = traits, domain, params
@@ -276,7 +278,16 @@ class LBM:
         self._inlets, self._outlets = list(inlets), list(outlets)
         self._cdom = domain.as_c()
         h = C.c_void_p()
-        if instances is None:
+        if ghosts is not None:
+            if instances is not None:
+                raise ValueError("ensembles take no ghost sites")
+            self._ghosts = ghosts
+            self._cghost = ghosts.as_c()
+            N.check(N.lib().hlb_create_subdomain(
+                C.byref(p), C.byref(self._cdom), C.byref(self._cghost),
+                _iolet_array(self._inlets), len(self._inlets),
+                _iolet_array(self._outlets), len(self._outlets), C.byref(h)))
+        elif instances is None:
             N.check(N.lib().hlb_create(C.byref(p), C.byref(self._cdom),
                                        _iolet_array(self._inlets), len(self._inlets),
                                        _iolet_array(self._outlets), len(self._outlets),
@@ -318,6 +329,14 @@ class LBM:
         ms = C.c_double()
         N.check(N.lib().hlb_time_steps(self._h, int(n), C.byref(ms)))
         return ms.value
+
+    def copy_ceiling(self, reps: int = 20) -> dict:
+        """Fastest device copy of the state arrays (hlb_copy_ceiling): read +
+        write GB/s, ms and bytes per copy, and the copy kernel's launch."""
+        r = N.hlb_copy_result()
+        N.check(N.lib().hlb_copy_ceiling(self._h, int(reps), C.byref(r)))
+        return {"GBs": r.bytes_per_s / 1e9, "ms": r.ms, "bytes": r.bytes,
+                "threads": r.threads, "xcd_remap": r.xcd_remap, "width": r.width}
 
     # FieldData export/import
     def get_distributions(self) -> np.ndarray:
@@ -458,3 +477,31 @@ class LBM:
 
     def synchronize(self) -> None:
         N.check(N.lib().hlb_synchronize(self._h))
+
+
+class HaloPlan:
+    """The values `dst`'s ghost sites need from `src` every step (the
+    reference's FieldData::SendAndReceive + CopyReceived between two ranks),
+    moved on the device: exchange() after each step of both subdomains."""
+
+    def __init__(self, src: LBM, dst: LBM):
+        h = C.c_void_p()
+        N.check(N.lib().hlb_halo_plan_create(src._h, dst._h, C.byref(h)))
+        self._h, self._keep = h, (src, dst)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            N.lib().hlb_halo_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def count(self) -> int:
+        return int(N.lib().hlb_halo_plan_count(self._h))
+
+    def exchange(self) -> None:
+        N.check(N.lib().hlb_halo_exchange(self._h))
+
+    def last_times(self) -> tuple:
+        a, b = C.c_double(), C.c_double()
+        N.check(N.lib().hlb_halo_last_times(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value

@@ -541,10 +541,14 @@ def predict_fields(net: "UNet", specs, ens, side: int, period: int, phase: float
 
 
 def evaluate(net: "UNet", seeds, side: int, period: int, tol: float = 1e-4,
-             max_cycles: int = 80, regime: str = "slow", log=print) -> dict:
+             max_cycles: int = 80, regime: str = "slow", log=print,
+             start: str = "feq") -> dict:
     """Held-out vessels run to the periodic state from rest (the plain
     method) and from the network's prediction; returns the cycles and wall
-    times of both and the difference of the converged fields."""
+    times of both and the difference of the converged fields.  start "feq":
+    the equilibrium of the predicted field (on the device); "ce": with the
+    Chapman-Enskog non-equilibrium part of its velocity gradient
+    (chapman_enskog_f, on the host)."""
     import time as _time
     _torch_cuda()
     specs = [run_spec(s, side, regime) for s in seeds]
@@ -563,7 +567,12 @@ def evaluate(net: "UNet", seeds, side: int, period: int, tol: float = 1e-4,
     sdfs = signed_distances(specs, side)
     t_sdf = _time.perf_counter() - t0
     rho, mom = predict_fields(net, specs, ens, side, period, 0.0, sdfs)
-    warm.set_equilibrium_fields(rho, mom)
+    if start == "ce":
+        r_h = rho.cpu().numpy()
+        warm.set_distributions(chapman_enskog_f(ens, specs, side, r_h,
+                                                mom.cpu().numpy() / r_h[:, None]))
+    else:
+        warm.set_equilibrium_fields(rho, mom)
     warm.SetTimeStep(1)
     t_pred = _time.perf_counter() - t0
     c_warm, n_warm = run_to_periodic(warm, ens, period, tol, max_cycles)
@@ -613,6 +622,87 @@ def evaluate(net: "UNet", seeds, side: int, period: int, tol: float = 1e-4,
     return res
 
 
+def chapman_enskog_f(ens, specs, side: int, rho: np.ndarray, u: np.ndarray) -> np.ndarray:
+    """Distributions [n, 19] of a start with the first-order non-equilibrium
+    part: f_i = feq_i(rho, u) - tau w_i rho / Cs2 (c_i c_i - Cs2 I) : S, with
+    S the symmetric velocity gradient by central differences on each
+    vessel's crop grid (solid voxels at rest) and tau the vessel's relaxation
+    time (the Chapman-Enskog expansion of the lattice BGK equation)."""
+    from .lb import D3Q19
+    c = D3Q19.VECTORS.astype(np.float64)
+    w = D3Q19.EQMWEIGHTS
+    cs2 = 1.0 / 3.0
+    f = np.empty((len(rho), 19))
+    for s, inst in zip(specs, ens.instances):
+        sl = inst.sites
+        ix = _crop_index(s, ens.domain.coords[sl], inst.offset, side)
+        grid = np.zeros((side ** 3, 3))
+        grid[ix] = u[sl]
+        grid = grid.reshape(side, side, side, 3)
+        g = np.stack(np.gradient(grid, axis=(0, 1, 2)), 0)   # [a, x, y, z, b] = d_a u_b
+        du = g.reshape(3, -1, 3)[:, ix, :].transpose(1, 0, 2)  # [site, a, b]
+        S = 0.5 * (du + du.transpose(0, 2, 1))
+        r, v = rho[sl], u[sl]
+        cu = v @ c.T
+        feq = w * r[:, None] * (1.0 + cu / cs2 + 0.5 * (cu / cs2) ** 2 -
+                                0.5 * (v * v).sum(1)[:, None] / cs2)
+        qs = np.einsum("ia,ib,sab->si", c, c, S) - cs2 * np.trace(S, axis1=1, axis2=2)[:, None]
+        f[sl] = feq - s.tau * w * r[:, None] / cs2 * qs
+    return f
+
+
+def epsilon_study(seeds, side: int, period: int, epsilons=(0.0, 1e-3, 1e-2, 1e-1),
+                  tol: float = 1e-4, max_cycles: int = 80, regime: str = "slow",
+                  log=print) -> dict:
+    """How many cycles a start of a given accuracy saves, without the
+    network.  The plain run from rest gives every held-out vessel's periodic
+    state; runs then start at the phase of step 1 from
+      * "exact_f": that state's distributions themselves;
+      * eps: the equilibrium of its density and velocity with relative error
+        eps (rho - 1 and u scaled by 1 - eps; eps = 0 is the exact
+        macroscopic field, whose start still lacks the non-equilibrium part);
+      * ce_eps: the same field with the Chapman-Enskog non-equilibrium part
+        of its velocity gradient (chapman_enskog_f);
+    and run to the periodic state.  Returns per-vessel cycles of every
+    start and the ratios to the plain run."""
+    _torch_cuda()
+    specs = [run_spec(s, side, regime) for s in seeds]
+    ens = _ensemble_of(specs, period)
+    cold = ens.solver(storage="fp32")
+    c_cold, n_cold = run_to_periodic(cold, ens, period, tol, max_cycles)
+    # the periodic state at the phase of step 1 (t = 1 mod period)
+    cold.step((n_cold + 1) * period + 1 - cold.GetTimeStep())
+    f_exact = cold.get_distributions()
+    rho, u = cold.macroscopic()
+    del cold
+    ok = c_cold > 0
+    res = dict(regime=regime, side=side, period=period, tol=tol, instances=len(specs),
+               cycles_cold=c_cold.tolist(), starts={})
+
+    def run(label, setup):
+        lbm = ens.solver(storage="fp32")
+        setup(lbm)
+        lbm.SetTimeStep(1)
+        c, _ = run_to_periodic(lbm, ens, period, tol, max_cycles)
+        good = ok & (c > 0)
+        ratio = c_cold[good] / c[good] if good.any() else np.array([np.nan])
+        res["starts"][label] = dict(cycles=c.tolist(), ratio_median=float(np.median(ratio)),
+                                    ratio_total=float(c_cold[good].sum() / max(c[good].sum(), 1)))
+        log(f"[warmstart] epsilon study {label}: cycles median {np.median(c[good]):.0f} "
+            f"(plain {np.median(c_cold[ok]):.0f}), per-vessel ratio median "
+            f"{res['starts'][label]['ratio_median']:.2f}")
+
+    run("exact_f", lambda lbm: lbm.set_distributions(f_exact))
+    for eps in epsilons:
+        r_e = 1.0 + (rho - 1.0) * (1.0 - eps)
+        u_e = u * (1.0 - eps)
+        m_e = u_e * r_e[:, None]
+        run(f"{eps:g}", lambda lbm, r_e=r_e, m_e=m_e: lbm.set_equilibrium_fields(r_e, m_e))
+        f_ce = chapman_enskog_f(ens, specs, side, r_e, u_e)
+        run(f"ce_{eps:g}", lambda lbm, f_ce=f_ce: lbm.set_distributions(f_ce))
+    return res
+
+
 def main(argv=None) -> int:
     import argparse
     import json
@@ -635,11 +725,24 @@ def main(argv=None) -> int:
     ap.add_argument("--save-weights", default="")
     ap.add_argument("--load-weights", default="",
                     help="evaluate saved parameters (no data generation or training)")
+    ap.add_argument("--start", choices=["feq", "ce"], default="feq",
+                    help="warm start from the predicted field's equilibrium, or with its "
+                         "Chapman-Enskog non-equilibrium part")
+    ap.add_argument("--epsilon-study", action="store_true",
+                    help="cycles saved by starts of known accuracy (no network)")
     a = ap.parse_args(argv)
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
     _torch_cuda()
+    if a.epsilon_study:
+        res = epsilon_study(list(range(100000, 100000 + a.eval_runs)), a.side, a.eval_period,
+                            tol=a.tol, max_cycles=a.max_cycles, regime=a.regime, log=log)
+        line = json.dumps(res)
+        print(line, flush=True)
+        if a.out:
+            open(a.out, "w").write(line + "\n")
+        return 0
     net = UNet(a.side, 1)
     info = dict(side=a.side)
     if a.load_weights:
@@ -664,7 +767,8 @@ def main(argv=None) -> int:
                     data_seconds=t_data, train_seconds=t_train, forward_ms=f_ms,
                     backward_ms=b_ms, train_cycles_median=float(np.median(train_cycles)))
     res = evaluate(net, list(range(100000, 100000 + a.eval_runs)), a.side, a.eval_period,
-                   a.tol, a.max_cycles, a.regime, log)
+                   a.tol, a.max_cycles, a.regime, log, a.start)
+    res["start"] = a.start
     res.update(info)
     line = json.dumps(res)
     print(line, flush=True)

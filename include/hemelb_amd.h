@@ -280,6 +280,26 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best,
 int hlb_snapshot_velocity(hlb_solver* s);
 int hlb_velocity_change(hlb_solver* s, double reference_speed, double* out);
 
+/* Device-copy ceiling of the step's traffic (a measurement aid; no
+ * counterpart in the reference): the solver's Q state arrays copied from the
+ * current state buffer into the other one (free between steps, so the state
+ * is kept), reps times per variant, one word per thread and array with the
+ * loads issued before streamed stores.  Variants: words of the storage's
+ * element size (4 bytes at least) or 16 bytes, 128- or 256-thread
+ * workgroups, the hardware workgroup order or runs of 8 workgroups per XCD.
+ * Reports the fastest: read + write bytes per second, milliseconds per copy,
+ * bytes per copy and its launch. */
+typedef struct {
+  double bytes_per_s;
+  double ms;
+  int64_t bytes;
+  int32_t threads;
+  int32_t xcd_remap;
+  int32_t width;
+  int32_t reserved;
+} hlb_copy_result;
+int hlb_copy_ceiling(hlb_solver* s, int32_t reps, hlb_copy_result* out);
+
 /* Halo exchange for a subdomain (Code/geometry/FieldData.cc:27-49): pack the
  * pending post-collision distributions of the given (site, direction) pairs
  * (the values the reference sends to the neighbouring rank) into a host
@@ -292,6 +312,44 @@ int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count,
  * FieldData::SendAndReceive / CopyReceived, FieldData.cc:27-49). */
 int hlb_halo_unpack(hlb_solver* s, const int64_t* pairs, int64_t count,
                     const double* host_buffer);
+
+/* ---- Subdomains and halo plans ------------------------------------------
+ * One rank's subdomain with ghost sites (the reference's decomposed Domain:
+ * edge sites and the shared distributions of Domain.cc:166-179 and 247-284).
+ * `ghosts` lists the sites of neighbouring subdomains that this domain's
+ * sites pull from across the cut (coords, wall_mask and iolet_mask are read;
+ * the other arrays may be NULL).  Ghost sites get state slots that the step
+ * reads and a halo exchange fills; the step never updates them.  The block
+ * and pull-table layouts support ghosts; ensembles do not. */
+int hlb_create_subdomain(const hlb_params* params, const hlb_domain* domain,
+                         const hlb_domain* ghosts, const hlb_iolet* inlets, int32_t n_inlets,
+                         const hlb_iolet* outlets, int32_t n_outlets, hlb_solver** out);
+
+/* A halo plan carries src's post-collision values into dst's ghost slots:
+ * every (ghost site g of dst owned by src, direction e) with g + c_e a site
+ * of dst and link e of g not cut -- what FieldData::SendAndReceive sends and
+ * CopyReceived places (FieldData.cc:27-49) in the reference's PreSend /
+ * PreReceive / PostReceive sequence (lb.hpp:175-251).  Both solvers are
+ * subdomains on one device with the same lattice and storage.  The indices
+ * are computed once; an exchange allocates nothing and does not block the
+ * host.  Run it after every step of both solvers (the first step included)
+ * and before the next step or export of dst. */
+typedef struct hlb_halo_plan hlb_halo_plan;
+int hlb_halo_plan_create(hlb_solver* src, hlb_solver* dst, hlb_halo_plan** out);
+void hlb_halo_plan_destroy(hlb_halo_plan* plan);
+/* Number of values one exchange moves (each one stored word: 8, 4 or 2
+ * bytes for FP64, FP32, FP16 storage). */
+int64_t hlb_halo_plan_count(const hlb_halo_plan* plan);
+/* pack on src's stream into the plan's device buffer, an event, unpack on
+ * dst's stream. */
+int hlb_halo_exchange(hlb_halo_plan* plan);
+/* The two halves for a caller that moves the buffer itself (e.g. between
+ * GPUs): pack into / unpack from a device buffer of count stored words, on
+ * src's / dst's stream; the caller orders the unpack after the pack. */
+int hlb_halo_pack_device(hlb_halo_plan* plan, void* device_buffer);
+int hlb_halo_unpack_device(hlb_halo_plan* plan, const void* device_buffer);
+/* Device milliseconds of the last hlb_halo_exchange's pack and unpack. */
+int hlb_halo_last_times(hlb_halo_plan* plan, double* pack_ms, double* unpack_ms);
 
 /* Device timing of n steps with HIP events on the solver's stream
  * (milliseconds, launch gaps included). */

@@ -65,9 +65,10 @@ def conv3d_backward(x, w, dy, bf16=False):
     xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (1, 1), (0, 0)))
     dxp = np.zeros_like(xp)
     dw = np.zeros_like(w)
+    dy2 = dy.reshape(-1, dy.shape[-1])
     for t, (kd, kh, kw) in enumerate(TAPS):
         win = xp[:, kd:kd + D, kh:kh + H, kw:kw + W, :]
-        dw[t] = np.einsum("ndhwo,ndhwi->oi", dy, win)
+        dw[t] = dy2.T @ np.ascontiguousarray(win).reshape(-1, Ci)
         dxp[:, kd:kd + D, kh:kh + H, kw:kw + W, :] += dy @ w[t]
     dx = dxp[:, 1:-1, 1:-1, 1:-1, :]
     db = dy.sum(axis=(0, 1, 2, 3))
@@ -154,7 +155,7 @@ def backward(p, acts, dout, bf16=False):
     """Parameter gradients of the loss whose gradient wrt the output is dout."""
     g = {}
     d1 = acts["head.in"]
-    g["head.w"] = np.einsum("ndhwc,ndhwk->ck", dout, d1)
+    g["head.w"] = dout.reshape(-1, dout.shape[-1]).T @ d1.reshape(-1, d1.shape[-1])
     g["head.b"] = dout.sum(axis=(0, 1, 2, 3))
     dd1 = dout @ p["head.w"].astype(np.float64)
 

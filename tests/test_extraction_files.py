@@ -133,3 +133,55 @@ def test_checkpoint_round_trip_and_lookup(tmp_path):
     e = _FakeLBM(coords, 15, np.zeros((len(coords), 15)), 0)
     with pytest.raises(N.HemeLBError, match="binary64"):
         X.load_checkpoint(e, path)
+
+
+class _FakeFlow:
+    """The parts of LBM the property writer uses: fOld's density, velocity
+    and wall shear stress, the domain and the time step."""
+
+    def __init__(self, coords, rho, u, wss, wall):
+        from types import SimpleNamespace
+        self.domain = SimpleNamespace(coords=coords, n=coords.shape[0],
+                                      wall_mask=wall.astype(np.uint32))
+        self.rho, self.u, self.wss = rho, u, wss
+
+    def macroscopic(self):
+        return self.rho.copy(), self.u.copy()
+
+    def wall_shear_stress(self):
+        return self.wss.copy()
+
+    def GetTimeStep(self):
+        return 7
+
+
+def test_property_file_pressure_offset_is_the_reference_pressure(tmp_path):
+    """The header's pressure offset is the UnitConverter's reference
+    pressure, so stored value + offset gives ConvertPressureToPhysicalUnits
+    of the density (LocalPropertyOutput.cc, LbDataSourceIterator.cc:34-38)."""
+    from hemelb_amd import lb as L
+    grid = _dummy_grid()
+    rng = np.random.default_rng(5)
+    n = len(grid)
+    wall = rng.random(n) < 0.3
+    flow = _FakeFlow(grid, 1.0 + 1e-3 * rng.standard_normal(n), 1e-3 * rng.random((n, 3)),
+                     np.where(wall, 1e-4 * rng.random(n), X.NO_VALUE), wall)
+    units = L.UnitConverter(6.25e-5, 5e-5, 1000.0, 80.0)
+    path = str(tmp_path / "props.xtr")
+    w = X.property_writer(flow, path, units)
+    assert X.write_properties(w, flow, units) == 7
+    w.close()
+    raw = open(path, "rb").read()
+    # field header of "pressure": name (4 + 8 bytes), n, type, n_offsets,
+    # then the offset as an XDR float
+    fh = raw[60:]
+    assert np.frombuffer(fh[:4], ">u4")[0] == 8 and fh[4:12] == b"pressure"
+    assert np.frombuffer(fh[12:24], ">u4").tolist() == [1, 0, 1]
+    offset = float(np.frombuffer(fh[24:28], ">f4")[0])
+    assert offset == 80.0
+    hlen = 60 + int(np.frombuffer(raw[56:60], ">u4")[0])
+    site = np.dtype([("x", ">u4", (3,)), ("p", ">f4"), ("u", ">f4", (3,)), ("s", ">f4")])
+    rec = _decode_records(raw, hlen, n, site)[0]
+    want = units.ConvertPressureToPhysicalUnits(flow.rho * L.Cs2)
+    got = rec["sites"]["p"].astype(np.float64) + offset
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-5 * np.abs(want).max())

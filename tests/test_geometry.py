@@ -153,3 +153,46 @@ def test_read_rejects_bad_files(tmp_path):
         Geometry.read_gmy(str(p))
     with pytest.raises(N.HemeLBError):
         Geometry.read_gmy(str(tmp_path / "missing.gmy"))
+
+
+def test_read_rejects_block_count_mismatch(tmp_path):
+    """A block whose fluid sites differ from the count its header entry
+    declares is malformed (GeometryReader.cc:248-281 reads the counts)."""
+    raw = bytearray(open(os.path.join(GOLDEN, "four_cube.gmy"), "rb").read())
+    # preamble: 3 magic/version words, 3 block counts, block size, padding
+    nblocks = int(np.prod(np.frombuffer(bytes(raw[12:24]), ">u4")))
+    hdr = np.frombuffer(bytes(raw[32:32 + 12 * nblocks]), ">u4").reshape(nblocks, 3)
+    b = int(np.flatnonzero(hdr[:, 0])[0])
+    bad = bytearray(raw)
+    bad[32 + 12 * b:32 + 12 * b + 4] = int(hdr[b, 0] + 1).to_bytes(4, "big")
+    p = tmp_path / "bad_count.gmy"
+    p.write_bytes(bytes(bad))
+    with pytest.raises(N.HemeLBError, match="fluid sites"):
+        Geometry.read_gmy(str(p))
+
+
+def test_split_domain_ghosts_are_the_pull_sources_across_the_cut():
+    """Every site a subdomain pulls from along a fluid link is either its
+    own or one of its ghosts, and every ghost is such a source (checked by
+    brute force over all sites and directions)."""
+    from hemelb_amd.geometry import split_domain
+    from hemelb_amd.lb import LATTICES
+    d = Geometry.vessel_tree(1, 3, 6.0).project(19)
+    c = LATTICES[19].VECTORS
+    pieces = split_domain(d, 3)
+    assert sum(p[0].n for p in pieces) == d.n
+    assert np.array_equal(np.sort(np.concatenate([p[1] for p in pieces])), np.arange(d.n))
+    where = {tuple(x): k for k, x in enumerate(d.coords)}
+    for own, idx, ghosts in pieces:
+        mine = set(map(tuple, own.coords))
+        need = set()
+        for x in own.coords:
+            for e in range(1, 19):
+                y = tuple(x - c[e])
+                j = where.get(y)
+                if j is None or y in mine:
+                    continue
+                if not ((d.wall_mask[j] | d.iolet_mask[j]) >> (e - 1)) & 1:
+                    need.add(y)
+        assert need == set(map(tuple, ghosts.coords))
+        assert need and not (need & mine)

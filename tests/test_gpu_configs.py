@@ -1,0 +1,136 @@
+"""The benchmark workloads themselves, at their full sizes, on the GPU.
+
+* Small cylinder (R = 12, L = 96, D3Q15 LBGK, SBB, Nash cosine inlet; 6000
+  steps = three inlet periods): bit-exact with the oracle at three points of
+  the run, the last after all 6000 steps.
+* Vessel tree with MRT (7.93 M sites, D3Q19, BFL, Ladd/Womersley + 16 Nash
+  outlets): binary64 bit-exact with the oracle over 3 steps; the binary32
+  and binary16 storage forms against binary64 after 10000 steps, with the
+  bounds written in the test.
+* The 96-vessel stenosed ensemble (binary32 storage, per-vessel tau, 51.9 M
+  sites): autotuned, 4000 steps, every vessel stable, and the smallest,
+  median and largest vessel bitwise equal to the same vessel run alone.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from hemelb_amd import extraction as X
+from hemelb_amd import lb as L
+from hemelb_amd import workloads as WL
+from tests import cases as K
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(orc, gpu, label):
+    assert gpu.GetTimeStep() == orc.time()
+    fo, fg = orc.get_f(), gpu.get_distributions()
+    bad = int(np.count_nonzero(fo != fg))
+    print(f"[configs] {label}: t={orc.time()} {bad} mismatches", flush=True)
+    assert bad == 0, label
+
+
+@pytest.mark.timeout(300)
+def test_small_cylinder_6000_steps_bitexact():
+    w = WL.small_cylinder()
+    assert w.steps == 6000
+    orc = K.workload_oracle(w)
+    gpu = w.solver(storage="fp64")
+    print(f"[configs] small cylinder: {w.domain.n} sites", flush=True)
+    done = 0
+    for n in (1, 1999, 4000):
+        t0 = time.time()
+        orc.step(n)
+        gpu.step(n)
+        done += n
+        _compare(orc, gpu, f"small cylinder after {done} steps ({time.time() - t0:.1f}s)")
+    assert done == w.steps
+    # three inlet periods have set the fluid moving
+    m = gpu.monitor()
+    assert m["stable"] and m["velocity_max"] > 1e-4
+
+
+@pytest.fixture(scope="module")
+def tree_mrt():
+    t0 = time.time()
+    w = WL.vessel_tree("MRT")
+    print(f"[configs] MRT tree: {w.domain.n} sites ({time.time() - t0:.1f}s)", flush=True)
+    return w
+
+
+@pytest.mark.timeout(600)
+def test_tree_mrt_bitexact_with_oracle(tree_mrt):
+    orc = K.workload_oracle(tree_mrt)
+    gpu = tree_mrt.solver(storage="fp64")
+    for n in (1, 2):
+        orc.step(n)
+        gpu.step(n)
+        _compare(orc, gpu, "MRT tree")
+
+
+# Relative errors of the reduced-precision forms against binary64 after
+# 10000 steps: velocity (L2 and max, relative to the binary64 field),
+# pressure (relative to its range) and wall shear stress (over wall sites).
+TREE_BOUNDS = {"fp32": dict(velocity=(1e-5, 1e-5), pressure=(1e-5, 1e-5),
+                            wall_shear_stress=(1e-5, 2e-5)),
+               "fp16": dict(velocity=(2e-2, 5e-2), pressure=(2e-2, 5e-2),
+                            wall_shear_stress=(2e-2, 5e-2))}
+
+
+@pytest.mark.timeout(600)
+def test_tree_mrt_reduced_precision_at_10000_steps(tree_mrt):
+    steps = tree_mrt.steps
+    assert steps == 10000
+    fields = {}
+    for storage in ("fp64", "fp32", "fp16"):
+        t0 = time.time()
+        lbm = tree_mrt.solver(storage=storage)
+        lbm.step(steps)
+        assert lbm.monitor()["stable"], storage
+        fields[storage] = X.extract(lbm)
+        del lbm
+        print(f"[configs] MRT tree {storage}: {steps} steps ({time.time() - t0:.1f}s)",
+              flush=True)
+    for storage, bounds in TREE_BOUNDS.items():
+        err = X.field_errors(fields[storage], fields["fp64"])
+        for name, (l2, linf) in bounds.items():
+            e = err[name]
+            print(f"[configs] MRT tree {storage} {name}: L2 {e['l2_rel']:.3e} "
+                  f"Linf {e['linf_rel']:.3e}", flush=True)
+            assert e["l2_rel"] <= l2 and e["linf_rel"] <= linf, (storage, name, e)
+
+
+@pytest.mark.timeout(900)
+def test_stenosed_ensemble_96_autotuned_4000_steps():
+    t0 = time.time()
+    e = WL.stenosed_ensemble(96)
+    assert len(e.instances) == 96 and e.steps == 4000
+    lbm = e.solver(storage="fp32")
+    print(f"[configs] ensemble: {e.domain.n} sites in 96 vessels "
+          f"({time.time() - t0:.1f}s)", flush=True)
+    shape = lbm.autotune(steps=10)
+    print(f"[configs] ensemble launch shape {shape}", flush=True)
+    t0 = time.time()
+    lbm.step(e.steps)
+    mons = lbm.monitor_instances()
+    print(f"[configs] ensemble 4000 steps ({time.time() - t0:.1f}s)", flush=True)
+    assert all(m["stable"] for m in mons)
+    sizes = np.array([i.sites.stop - i.sites.start for i in e.instances])
+    assert [m["sites"] for m in mons] == sizes.tolist()
+    order = np.argsort(sizes, kind="stable")
+    picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    f_ens = lbm.get_distributions()
+    del lbm
+    for k in picks:
+        inst = e.instances[k]
+        alone = L.LBM(e.traits, e.instance_domain(k), L.LbmParameters(tau=inst.tau),
+                      e.inlets[inst.inlets], e.outlets[inst.outlets], storage="fp32",
+                      rho0=1.0)
+        alone.SetInitialConditions(1.0)
+        alone.step(e.steps)
+        same = np.array_equal(alone.get_distributions(), f_ens[inst.sites])
+        print(f"[configs] vessel {k} ({sizes[k]} sites, tau {inst.tau:.3f}) alone: "
+              f"{'bitwise equal' if same else 'DIFFERS'}", flush=True)
+        assert same, k

@@ -287,3 +287,21 @@ def test_split_launch_is_bitexact(layout):
         orc.step(n)
         gpu.step(n)
         np.testing.assert_array_equal(gpu.get_distributions(), orc.get_f())
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+@pytest.mark.parametrize("layout", ["blocks", "compact_blocks"])
+def test_split_launch_reduced_storage_is_bitexact(storage, layout):
+    """The split launch (a tuner candidate for every storage form) against
+    the single launch in the reduced-precision forms, whose boundary path
+    differs (binary16 loads the BFL own values on every lane)."""
+    case = K.tree_case(3, 6.0, seed=1)
+    print(f"[parity] split launch {storage} layout={layout} n={case.domain.n}", flush=True)
+    one = case.product(storage=storage, layout=layout)
+    one.set_launch_shape(threads=128, xcd_remap=8, split=0)
+    two = case.product(storage=storage, layout=layout)
+    two.set_launch_shape(threads=128, xcd_remap=8, split=1)
+    for n in (1, 2, 17):
+        one.step(n)
+        two.step(n)
+        np.testing.assert_array_equal(two.get_distributions(), one.get_distributions())

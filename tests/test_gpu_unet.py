@@ -29,9 +29,17 @@ def _case(side, batch, chans, seed):
     return v, p, x, t, mask
 
 
-@pytest.mark.parametrize("side,batch,chans", [(16, 1, (16, 16, 16, 16)),
-                                              (16, 2, (16, 32, 32, 48)),
-                                              (32, 1, (16, 32, 64, 128))])
+@pytest.mark.parametrize("side,batch,chans", [
+    (16, 1, (16, 16, 16, 16)),
+    (16, 2, (16, 32, 32, 48)),
+    (32, 1, (16, 32, 64, 128)),
+    # 3 * 2^18 positions at the finest level: the weight gradient takes 384
+    # positions per chunk in 2048 chunks
+    (64, 3, (16, 16, 16, 16)),
+    # the warm start's own shape: a 128^3 crop through the full 16-128
+    # channel network (the oracle takes about a minute here)
+    pytest.param(128, 1, (16, 32, 64, 128), marks=pytest.mark.timeout(900)),
+])
 def test_forward_and_gradients_match_oracle(side, batch, chans):
     print(f"[unet] side={side} batch={batch} chans={chans}", flush=True)
     v, p, x, t, mask = _case(side, batch, chans, side + batch)

@@ -11,13 +11,21 @@ held-out vessels to the periodic state twice: from rest (the plain method)
 and from the network's prediction.
 
 The assertions:
-* the training loss falls;
+* the training loss falls: the mean over the last quarter of the epochs is
+  below the mean over the first quarter;
 * the prediction is closer to the periodic field than the plain method's
   starting state (relative error 1, the field itself);
+* the start never costs cycles: the held-out vessels need no more cycles in
+  total from the prediction than from rest;
 * both runs converge to the same periodic field.
 
-The measured cycle and wall-time ratios are printed. They are reported, not
-asserted: at this training budget they are close to 1.
+The measured cycle and wall-time ratios are printed; the >= 3x target is not
+asserted (at this training budget the ratio is close to 1).
+
+A second run uses the warm start's own specification, on a budget: 128^3
+crops of seeded stenosed cylinders and bifurcations (tau 0.6-1.0, inlet
+amplitude 0.5-2 mmHg), 16 training runs and 4 held-out vessels; its
+converged field must agree with the plain run's to 1e-4.
 """
 import json
 import os
@@ -47,7 +55,35 @@ def test_trained_network_beats_the_plain_start(tmp_path):
           f"{sum(res['cycles_cold'])} warm {sum(res['cycles_warm'])} (ratio "
           f"{res['cycle_ratio_total']:.2f}); wall {res['wall_cold_s']:.2f}s vs "
           f"{res['wall_warm_s']:.2f}s", flush=True)
-    assert res["losses"][-1] < res["losses"][0]
+    losses = res["losses"]
+    k = max(1, len(losses) // 4)
+    assert sum(losses[-k:]) / k < sum(losses[:k]) / k, losses
     assert not res["unstable_cold"] and not res["unstable_warm"]
     assert res["prediction_rel_l2_median"] < 1.0
+    assert res["cycle_ratio_total"] >= 1.0
     assert res["converged_field_median_rel_diff"] < 1e-4
+
+
+def _run(tmp_path, args, timeout):
+    out = str(tmp_path / "ws.json")
+    cmd = [sys.executable, "-u", "-m", "hemelb_amd.warmstart"] + args + ["--out", out]
+    print("[warmstart] " + " ".join(cmd[3:]), flush=True)
+    r = subprocess.run(cmd, cwd=ROOT, timeout=timeout)
+    assert r.returncode == 0
+    return json.load(open(out))
+
+
+@pytest.mark.timeout(900)
+def test_warm_start_at_128_cube_crops(tmp_path):
+    res = _run(tmp_path, ["--side", "128", "--regime", "fast", "--train-runs", "16",
+                          "--batch-runs", "16", "--eval-runs", "4", "--periods", "1000",
+                          "--eval-period", "1000", "--epochs", "50", "--lr", "3e-4",
+                          "--train-seconds", "40", "--max-cycles", "40"], 840)
+    print(f"[warmstart] 128^3: prediction error {res['prediction_rel_l2_median']:.3f}; "
+          f"cycles cold {sum(res['cycles_cold'])} warm {sum(res['cycles_warm'])}; wall "
+          f"{res['wall_cold_s']:.2f}s vs {res['wall_warm_s']:.2f}s; converged field max "
+          f"difference {res['converged_field_max_rel_diff']:.2e}", flush=True)
+    assert res["side"] == 128 and res["instances"] == 4
+    assert not res["unstable_cold"] and not res["unstable_warm"]
+    assert all(c > 0 for c in res["cycles_cold"]) and all(c > 0 for c in res["cycles_warm"])
+    assert res["converged_field_max_rel_diff"] < 1e-4

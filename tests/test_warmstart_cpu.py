@@ -62,3 +62,33 @@ def test_parameter_layout_round_trip():
     np.testing.assert_array_equal(WS.pack_params(p, chans), v)
     n = sum(27 * co * ci + co for _, ci, co in WS.layer_shapes(chans)) + 4 * 16 + 4
     assert v.size == n
+
+
+def test_chapman_enskog_start_carries_the_viscous_stress():
+    """A simple shear flow u = (g y, 0, 0): the non-equilibrium part of the
+    start has no mass or momentum, and its second moment is the BGK viscous
+    stress -tau rho Cs2 (du_x/dy + du_y/dx) = -tau rho g / 3 (fourth-order
+    isotropy of the D3Q19 weights), away from the crop's faces."""
+    from types import SimpleNamespace
+    side, g, tau = 8, 1e-3, 0.8
+    xyz = np.array([(x, y, z) for x in range(side) for y in range(side) for z in range(side)],
+                   np.int32)
+    spec = SimpleNamespace(origin=(0, 0, 0), tau=tau)
+    inst = SimpleNamespace(sites=slice(0, len(xyz)), offset=(0, 0, 0))
+    ens = SimpleNamespace(domain=SimpleNamespace(coords=xyz), instances=[inst])
+    rho = np.full(len(xyz), 1.01)
+    u = np.zeros((len(xyz), 3))
+    u[:, 0] = g * xyz[:, 1]
+    f = WS.chapman_enskog_f(ens, [spec], side, rho, u)
+    from hemelb_amd.lb import D3Q19
+    c = D3Q19.VECTORS.astype(float)
+    w = D3Q19.EQMWEIGHTS
+    cu = u @ c.T
+    feq = w * rho[:, None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u * u).sum(1)[:, None])
+    fneq = f - feq
+    inner = ((xyz > 0) & (xyz < side - 1)).all(1)
+    np.testing.assert_allclose(fneq.sum(1), 0.0, atol=1e-15)
+    np.testing.assert_allclose(fneq @ c, 0.0, atol=1e-15)
+    pxy = (fneq * c[:, 0] * c[:, 1]).sum(1)
+    np.testing.assert_allclose(pxy[inner], -tau * 1.01 * g / 3.0, rtol=1e-12)
+    np.testing.assert_allclose((fneq * c[:, 0] * c[:, 0]).sum(1)[inner], 0.0, atol=1e-15)
