@@ -298,10 +298,11 @@ def main() -> int:
         cc = lbm.copy_ceiling(reps=20)
         roof["copy_ceiling"] = round(cc["GBs"], 1)
         roof["frac_of_copy"] = round(achieved / cc["GBs"], 4)
-        roof["copy_kernel"] = {k: cc[k] for k in ("threads", "xcd_remap", "width", "bytes")}
+        roof["copy_kernel"] = {k: cc[k] for k in ("threads", "xcd_remap", "width", "order",
+                                                  "bytes")}
         log(f"[bench] device copy of the {cc['bytes'] / 1e9:.2f} GB of state arrays: "
-            f"{cc['GBs']:.0f} GB/s ({cc['ms']:.3f} ms, {cc['threads']} threads, remap "
-            f"{cc['xcd_remap']}, {cc['width']}-byte words); the step reaches "
+            f"{cc['GBs']:.0f} GB/s ({cc['ms']:.3f} ms, {cc['order']} order, {cc['threads']} "
+            f"threads, remap {cc['xcd_remap']}, {cc['width']}-byte words); the step reaches "
             f"{achieved / cc['GBs']:.1%} of it")
     del lbm
     forms = None

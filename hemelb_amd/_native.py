@@ -70,7 +70,7 @@ class hlb_launch_shape(C.Structure):
 class hlb_copy_result(C.Structure):
     _fields_ = [("bytes_per_s", C.c_double), ("ms", C.c_double), ("bytes", C.c_int64),
                 ("threads", C.c_int32), ("xcd_remap", C.c_int32), ("width", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("order", C.c_int32)]
 
 
 HLBX_FLOAT, HLBX_DOUBLE, HLBX_INT32, HLBX_UINT32, HLBX_INT64, HLBX_UINT64 = range(6)

@@ -144,12 +144,13 @@ int launch_wall_shear_stress(int Q, int64_t nw, int64_t np, const double* soa,
 int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
                   const double* rho, const double* u, double* partials,
                   int nblocks, void* stream);
-// Device-copy ceiling: Q arrays of `bytes` bytes (array d at byte offset
-// d * stride_bytes) copied from `in` to `out` in words of `width` bytes
-// (4, 8 or 16), one word per thread and array, with the step's workgroup
-// size and workgroup-to-XCD remap.
-int launch_copy_arrays(int Q, int64_t stride_bytes, int64_t bytes, int width,
-                       const void* in, void* out, uint32_t threads, uint32_t remap,
+// Device-copy ceiling: Q arrays of `bytes` bytes copied from `in` to `out`
+// in words of `width` bytes (4, 8 or 16), one word per thread and array,
+// with the step's workgroup size and workgroup-to-XCD remap.  run_bytes 0:
+// array d at byte offset d * stride_bytes; else the block layout's order,
+// runs of run_bytes per direction and block.
+int launch_copy_arrays(int Q, int64_t stride_bytes, int64_t bytes, int64_t run_bytes,
+                       int width, const void* in, void* out, uint32_t threads, uint32_t remap,
                        void* stream);
 // Halo exchange of a subdomain pair: pack (pack != 0) copies
 // from[idx[k]] to to[k], unpack copies from[k] to to[idx[k]], for k < count,
@@ -197,12 +198,14 @@ int launch_velocity_change_ranges(int64_t nchunks, int64_t np, const int64_t* ra
 int launch_velocity_change(int64_t n, int64_t np, const double* u,
                            const double* u_ref, double inv_ref, double* partials,
                            int nblocks, void* stream);
-// Halo unpack: state[slot_index[i]] = in[i] (storage conversion applied).
-int launch_scatter_pairs(int64_t count, int64_t np, const int64_t* slot_index,
-                         const double* in, int storage, double rho0, const double* w,
-                         void* state, void* stream);
-int launch_gather_pairs(int64_t count, int64_t np, const int64_t* slot_index,
-                        const void* state, int storage, double rho0,
+// Halo unpack: state[slot_index[i]] = in[i] (storage conversion applied);
+// the direction of a state index is index / np, or (index / 512) % Q for the
+// block layout (blockwise != 0).
+int launch_scatter_pairs(int64_t count, int64_t np, int Q, int blockwise,
+                         const int64_t* slot_index, const double* in, int storage, double rho0,
+                         const double* w, void* state, void* stream);
+int launch_gather_pairs(int64_t count, int64_t np, int Q, int blockwise,
+                        const int64_t* slot_index, const void* state, int storage, double rho0,
                         const double* w, double* out, void* stream);
 
 }  // namespace hlb

@@ -422,9 +422,15 @@ struct SparseLayout {
     r.cut = bi != 0 ? 1u : 0u;  // only "is boundary" is needed here
     r.rec = bi - 1;
   }
+  // State index of direction d of slot x, and of the value streaming into
+  // slot E of the lane's site.
+  template <int Q>
+  static __device__ __forceinline__ int64_t at(const DevDomain& D, int d, int64_t x) {
+    return (int64_t)d * D.np + x;
+  }
   template <int Q, int E>
-  static __device__ __forceinline__ int64_t offset(const DevDomain& D, const SiteRef& r) {
-    return (int64_t)D.src[(int64_t)(E - 1) * D.np + r.x];
+  static __device__ __forceinline__ int64_t index(const DevDomain& D, const SiteRef& r) {
+    return (int64_t)E * D.np + (int64_t)D.src[(int64_t)(E - 1) * D.np + r.x];
   }
   static __device__ __forceinline__ int64_t compact(const DevDomain&, const SiteRef& r) {
     return r.x;
@@ -433,7 +439,9 @@ struct SparseLayout {
 };
 
 // Block layout: every 8^3 block holding fluid owns 512 slots (x-plane major,
-// z fastest); a wave covers one x plane of one block.  Neighbours come from
+// z fastest) and its Q directions are stored together, block after block
+// ([block][direction][512 slots]), so that one block's distributions are
+// one contiguous run of memory; a wave covers one x plane of one block.  Neighbours come from
 // slot arithmetic and the block's 27-entry neighbour table, which each wave
 // loads once; per-wave 64-bit masks say which lanes are fluid and which carry
 // a boundary record, whose wall and iolet masks mark the cut links.
@@ -465,8 +473,12 @@ struct BlockLayout {
     r.cut = ((bm >> r.lane) & 1ull) ? (D.b_wall[r.rec] | D.b_iolet[r.rec]) : 0u;
     return true;
   }
+  template <int Q>
+  static __device__ __forceinline__ int64_t at(const DevDomain&, int d, int64_t slot) {
+    return ((slot >> 9) * Q + d) * 512 + (slot & 511);
+  }
   template <int Q, int E>
-  static __device__ __forceinline__ int64_t offset(const DevDomain& D, const SiteRef& r) {
+  static __device__ __forceinline__ int64_t index(const DevDomain& D, const SiteRef& r) {
     constexpr int CX = Lat<Q>::c[E][0], CY = Lat<Q>::c[E][1], CZ = Lat<Q>::c[E][2];
     const int lx = (int)(r.w & 7u);
     const int ly = (int)(r.lane >> 3), lz = (int)(r.lane & 7u);
@@ -483,10 +495,10 @@ struct BlockLayout {
     const int32_t n01 = bx ? r.tb[i101] : r.tb[i001];
     const int32_t n11 = bx ? r.tb[i111] : r.tb[i011];
     const int32_t nb = by ? (bz ? n11 : n10) : (bz ? n01 : n00);
-    const int64_t src = ((int64_t)nb << 9) +
+    const int64_t src = ((int64_t)nb * Q + E) * 512 +
                         ((((lx - CX) & 7) << 6) | (((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
     constexpr int DP = inv_dir(E);
-    const int64_t own = r.x + (int64_t)(DP - E) * D.np;
+    const int64_t own = at<Q>(D, DP, r.x);
     return ((r.cut >> (DP - 1)) & 1u) ? own : src;
   }
   static __device__ __forceinline__ int64_t compact(const DevDomain& D, const SiteRef& r) {
@@ -553,8 +565,12 @@ struct CompactBlockLayout {
       b = 0;
     }
   }
+  template <int Q>
+  static __device__ __forceinline__ int64_t at(const DevDomain& D, int d, int64_t x) {
+    return (int64_t)d * D.np + x;
+  }
   template <int Q, int E>
-  static __device__ __forceinline__ int64_t offset(const DevDomain& D, const SiteRef& r) {
+  static __device__ __forceinline__ int64_t index(const DevDomain& D, const SiteRef& r) {
     constexpr int CX = Lat<Q>::c[E][0], CY = Lat<Q>::c[E][1], CZ = Lat<Q>::c[E][2];
     const int lx = (int)(r.w & 7u);
     const int ly = (int)(r.lane >> 3), lz = (int)(r.lane & 7u);
@@ -576,8 +592,7 @@ struct CompactBlockLayout {
     const uint32_t sl = (uint32_t)((((ly - CY) & 7) << 3) | ((lz - CZ) & 7));
     const int64_t src = site_of(m, b, sl);
     constexpr int DP = inv_dir(E);
-    const int64_t own = r.x + (int64_t)(DP - E) * D.np;
-    return ((r.cut >> (DP - 1)) & 1u) ? own : src;
+    return ((r.cut >> (DP - 1)) & 1u) ? at<Q>(D, DP, r.x) : at<Q>(D, E, src);
   }
   static __device__ __forceinline__ int64_t compact(const DevDomain&, const SiteRef& r) {
     return r.x;
@@ -653,11 +668,11 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
                                        const S* __restrict__ fin, const SiteRef& r,
                                        T (&f)[Q]) {
   using St = Store<S>;
-  const int64_t np = D.np, x = r.x;
-  f[0] = St::ld(fin + x);
+  const int64_t x = r.x;
+  f[0] = St::ld(fin + Lay::template at<Q>(D, 0, x));
   sfor<1, Q>([&](auto ei) {
     constexpr int e = decltype(ei)::value;
-    f[e] = St::ld(fin + (int64_t)e * np + Lay::template offset<Q, e>(D, r));
+    f[e] = St::ld(fin + Lay::template index<Q, e>(D, r));
   });
   if constexpr (BULK) return;  // a wave without boundary sites
   if (!r.cut) return;
@@ -708,9 +723,10 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
                         !((wm >> (e - 1)) & 1u);
       if constexpr (sizeof(S) >= 4) {
         opp[e] = T(0);
-        if (link && !((T)(double)q[dp - 1] < T(0.5))) opp[e] = St::ld(fin + (int64_t)e * np + x);
+        if (link && !((T)(double)q[dp - 1] < T(0.5)))
+          opp[e] = St::ld(fin + Lay::template at<Q>(D, e, x));
       } else {
-        opp[e] = St::ld(fin + (link ? (int64_t)e * np + x : (int64_t)dp * np + x));
+        opp[e] = St::ld(fin + Lay::template at<Q>(D, link ? e : dp, x));
       }
     });
     sfor<1, Q>([&](auto ei) {
@@ -803,7 +819,7 @@ __device__ __forceinline__ void step_site(const DevDomain& D, const StepArgs& A,
   update_stash<Q, T, BULK>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    store_out<S>(fout + (int64_t)d * D.np + r.x, f[d]);
+    store_out<S>(fout + Lay::template at<Q>(D, d, r.x), f[d]);
   });
 }
 
@@ -877,7 +893,7 @@ __global__ __launch_bounds__(256) void k_collide_only(DevDomain D, StepArgs A,
   T f[Q];
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    f[d] = Store<S>::ld(fin + (int64_t)d * D.np + r.x);
+    f[d] = Store<S>::ld(fin + Lay::template at<Q>(D, d, r.x));
   });
   StepArgs a = A;
   Lay::relax(D, r, a);
@@ -885,7 +901,7 @@ __global__ __launch_bounds__(256) void k_collide_only(DevDomain D, StepArgs A,
   update_stash<Q, T>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    Store<S>::st(fout + (int64_t)d * D.np + r.x, f[d]);
+    Store<S>::st(fout + Lay::template at<Q>(D, d, r.x), f[d]);
   });
 }
 
@@ -902,7 +918,7 @@ __global__ __launch_bounds__(256) void k_stream_only(DevDomain D, StepArgs A,
   if (from_post_stream) {
     sfor<0, Q>([&](auto di) {
       constexpr int d = decltype(di)::value;
-      f[d] = Store<S>::ld(fin + (int64_t)d * D.np + r.x);
+      f[d] = Store<S>::ld(fin + Lay::template at<Q>(D, d, r.x));
     });
   } else {
     gather<Q, S, T, Lay>(D, A, fin, r, f);
@@ -924,19 +940,23 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 template <int Q, typename W>
-__global__ __launch_bounds__(256) void k_copy_arrays(int64_t stride, int64_t words,
+__global__ __launch_bounds__(256) void k_copy_arrays(int64_t stride, int64_t words, int64_t run,
                                                      const W* __restrict__ in,
                                                      W* __restrict__ out, uint32_t remap) {
   const int64_t i = (int64_t)step_workgroup(remap) * blockDim.x + threadIdx.x;
   if (i >= words) return;
+  // run > 0: the block layout's order, runs of `run` words per direction
+  // and block; else arrays of `stride` words per direction
+  const int64_t base = run ? (i / run) * Q * run + (i % run) : i;
+  const int64_t step = run ? run : stride;
   W v[Q];
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    v[d] = __builtin_nontemporal_load(in + (int64_t)d * stride + i);
+    v[d] = __builtin_nontemporal_load(in + base + d * step);
   });
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    __builtin_nontemporal_store(v[d], out + (int64_t)d * stride + i);
+    __builtin_nontemporal_store(v[d], out + base + d * step);
   });
 }
 
@@ -968,7 +988,7 @@ __global__ __launch_bounds__(256) void k_import(DevDomain D, const double* aos,
     constexpr int d = decltype(di)::value;
     double v = aos[c * Q + d];
     if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
-    Store<S>::st(state + (int64_t)d * D.np + r.x, (typename Store<S>::T)v);
+    Store<S>::st(state + Lay::template at<Q>(D, d, r.x), (typename Store<S>::T)v);
   });
 }
 
@@ -988,20 +1008,21 @@ __global__ __launch_bounds__(256) void k_import_equilibrium(DevDomain D, const d
     constexpr int d = decltype(di)::value;
     double v = fe.template eq<Q, d>();
     if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
-    Store<S>::st(state + (int64_t)d * D.np + r.x, (typename Store<S>::T)v);
+    Store<S>::st(state + Lay::template at<Q>(D, d, r.x), (typename Store<S>::T)v);
   });
 }
 
 template <int Q, typename S>
-__global__ void k_fill(int64_t nslots, int64_t np, const double* vals, S* state,
-                       double rho0) {
+__global__ void k_fill(int64_t nslots, int64_t np, int blockwise, const double* vals,
+                       S* state, double rho0) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nslots) return;
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
     double v = vals[d];
     if constexpr (kShifted<S>) v -= Lat<Q>::w[d] * rho0;
-    Store<S>::st(state + (int64_t)d * np + x, (typename Store<S>::T)v);
+    const int64_t i = blockwise ? ((x >> 9) * Q + d) * 512 + (x & 511) : (int64_t)d * np + x;
+    Store<S>::st(state + i, (typename Store<S>::T)v);
   });
 }
 
@@ -1098,7 +1119,7 @@ __global__ __launch_bounds__(256) void k_monitor(DevDomain D, StepArgs A,
     if (from_post_stream) {
       sfor<0, Q>([&](auto di) {
         constexpr int d = decltype(di)::value;
-        f[d] = Store<S>::ld(fin + (int64_t)d * D.np + r.x);
+        f[d] = Store<S>::ld(fin + Lay::template at<Q>(D, d, r.x));
       });
     } else {
       gather<Q, S, T, Lay>(D, A, fin, r, f);
@@ -1428,26 +1449,29 @@ __global__ __launch_bounds__(256) void k_traction(
 }
 
 template <typename S>
-__global__ void k_gather_pairs(int64_t count, int64_t np, const int64_t* slot,
-                               const S* state, double rho0, const double* w,
-                               double* out) {
+__global__ void k_gather_pairs(int64_t count, int64_t np, int Q, int blockwise,
+                               const int64_t* slot, const S* state, double rho0,
+                               const double* w, double* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const int64_t sidx = slot[i];
   double v = (double)Store<S>::ld(state + sidx);
-  if constexpr (kShifted<S>) v += w[sidx / np] * rho0;
+  const int d = blockwise ? (int)((sidx >> 9) % Q) : (int)(sidx / np);
+  if constexpr (kShifted<S>) v += w[d] * rho0;
   out[i] = v;
 }
 
 // Halo unpack: state[slot[i]] = in[i] (shifted storage stores in - w rho0).
 template <typename S>
-__global__ void k_scatter_pairs(int64_t count, int64_t np, const int64_t* slot,
-                                const double* in, double rho0, const double* w, S* state) {
+__global__ void k_scatter_pairs(int64_t count, int64_t np, int Q, int blockwise,
+                                const int64_t* slot, const double* in, double rho0,
+                                const double* w, S* state) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const int64_t sidx = slot[i];
   double v = in[i];
-  if constexpr (kShifted<S>) v -= w[sidx / np] * rho0;
+  const int d = blockwise ? (int)((sidx >> 9) % Q) : (int)(sidx / np);
+  if constexpr (kShifted<S>) v -= w[d] * rho0;
   Store<S>::st(state + sidx, (typename Store<S>::T)v);
 }
 
@@ -1493,7 +1517,7 @@ int launch_fill(const DevDomain& d, const StepArgs& a, const double* vals, void*
     using S = std::remove_pointer_t<decltype(s)>;
     constexpr int Q = decltype(q)::value;
     hipLaunchKernelGGL((k_fill<Q, S>), grid_for(d.np), dim3(256), 0, (hipStream_t)stream,
-                       d.np, d.np, vals, (S*)state, a.rho0);
+                       d.np, d.np, d.layout == 1 ? 1 : 0, vals, (S*)state, a.rho0);
     return (int)hipGetLastError();
   });
 }
@@ -1647,11 +1671,14 @@ int launch_reduce(int Q, int64_t n, int64_t np, const double* soa,
   return (int)hipGetLastError();
 }
 
-int launch_copy_arrays(int Q, int64_t stride_bytes, int64_t bytes, int width,
-                       const void* in, void* out, uint32_t threads, uint32_t remap,
+int launch_copy_arrays(int Q, int64_t stride_bytes, int64_t bytes, int64_t run_bytes,
+                       int width, const void* in, void* out, uint32_t threads, uint32_t remap,
                        void* stream) {
   if (width != 4 && width != 8 && width != 16) return (int)hipErrorInvalidValue;
-  if (stride_bytes % width || bytes % width) return (int)hipErrorInvalidValue;
+  if (stride_bytes % width || bytes % width || run_bytes % width)
+    return (int)hipErrorInvalidValue;
+  if (run_bytes && bytes % run_bytes) return (int)hipErrorInvalidValue;
+  const int64_t run = run_bytes / width;
   if (threads != 64 && threads != 128 && threads != 256) return (int)hipErrorInvalidValue;
   const int64_t words = bytes / width, stride = stride_bytes / width;
   uint64_t groups = ((uint64_t)words + threads - 1) / threads;
@@ -1663,7 +1690,7 @@ int launch_copy_arrays(int Q, int64_t stride_bytes, int64_t bytes, int width,
   auto go = [&](auto q, auto* w) {
     using W = std::remove_pointer_t<decltype(w)>;
     hipLaunchKernelGGL((k_copy_arrays<decltype(q)::value, W>), grid, block, 0,
-                       (hipStream_t)stream, stride, words, (const W*)in, (W*)out, remap);
+                       (hipStream_t)stream, stride, words, run, (const W*)in, (W*)out, remap);
   };
   if (Q == 15) {
     if (width == 4) go(std::integral_constant<int, 15>{}, (unsigned int*)nullptr);
@@ -1784,37 +1811,38 @@ int launch_velocity_change(int64_t n, int64_t np, const double* u,
   return (int)hipGetLastError();
 }
 
-int launch_gather_pairs(int64_t count, int64_t np, const int64_t* slot_index,
-                        const void* state, int storage, double rho0,
+int launch_gather_pairs(int64_t count, int64_t np, int Q, int blockwise,
+                        const int64_t* slot_index, const void* state, int storage, double rho0,
                         const double* w, double* out, void* stream) {
   if (storage == 0)
     hipLaunchKernelGGL(k_gather_pairs<double>, grid_for(count), dim3(256), 0,
-                       (hipStream_t)stream, count, np, slot_index,
+                       (hipStream_t)stream, count, np, Q, blockwise, slot_index,
                        (const double*)state, rho0, w, out);
   else if (storage == 1)
     hipLaunchKernelGGL(k_gather_pairs<float>, grid_for(count), dim3(256), 0,
-                       (hipStream_t)stream, count, np, slot_index,
+                       (hipStream_t)stream, count, np, Q, blockwise, slot_index,
                        (const float*)state, rho0, w, out);
   else
     hipLaunchKernelGGL(k_gather_pairs<_Float16>, grid_for(count), dim3(256), 0,
-                       (hipStream_t)stream, count, np, slot_index,
+                       (hipStream_t)stream, count, np, Q, blockwise, slot_index,
                        (const _Float16*)state, rho0, w, out);
   return (int)hipGetLastError();
 }
 
-int launch_scatter_pairs(int64_t count, int64_t np, const int64_t* slot_index,
-                         const double* in, int storage, double rho0, const double* w,
-                         void* state, void* stream) {
+int launch_scatter_pairs(int64_t count, int64_t np, int Q, int blockwise,
+                         const int64_t* slot_index, const double* in, int storage, double rho0,
+                         const double* w, void* state, void* stream) {
   if (storage == 0)
     hipLaunchKernelGGL(k_scatter_pairs<double>, grid_for(count), dim3(256), 0,
-                       (hipStream_t)stream, count, np, slot_index, in, rho0, w,
+                       (hipStream_t)stream, count, np, Q, blockwise, slot_index, in, rho0, w,
                        (double*)state);
   else if (storage == 1)
     hipLaunchKernelGGL(k_scatter_pairs<float>, grid_for(count), dim3(256), 0,
-                       (hipStream_t)stream, count, np, slot_index, in, rho0, w, (float*)state);
+                       (hipStream_t)stream, count, np, Q, blockwise, slot_index, in, rho0, w,
+                       (float*)state);
   else
     hipLaunchKernelGGL(k_scatter_pairs<_Float16>, grid_for(count), dim3(256), 0,
-                       (hipStream_t)stream, count, np, slot_index, in, rho0, w,
+                       (hipStream_t)stream, count, np, Q, blockwise, slot_index, in, rho0, w,
                        (_Float16*)state);
   return (int)hipGetLastError();
 }

@@ -347,6 +347,13 @@ struct hlb_solver {
     }
   }
 
+  // The block layout stores each block's directions together
+  // ([block][direction][512 slots]); the others one array per direction.
+  int blockwise() const { return p.layout == HLB_LAYOUT_BLOCKS ? 1 : 0; }
+  int64_t state_index(int d, int64_t slot) const {
+    return blockwise() ? ((slot >> 9) * Q + d) * 512 + (slot & 511) : d * np + slot;
+  }
+
   int ensure_scratch_soa() {
     if (!s_soa.p) HIP_OR_FAIL(s_soa.alloc((size_t)Q * npc));
     return HLB_OK;
@@ -1672,13 +1679,17 @@ int hlb_copy_ceiling(hlb_solver* s, int32_t reps, hlb_copy_result* out) {
   const void* src = s->state[s->cur];
   void* dst = s->state[s->cur ^ 1];
   hlb_copy_result best{};
+  for (int order : {0, 1})
   for (int width : {(int)std::max<int64_t>(esize, 4), 16}) {
-    if (bytes % width) continue;
+    // order 1: the block layout's order (runs of 512 slots per direction and
+    // block); order 0: one array per direction
+    const int64_t run = order ? 512 * esize : 0;
+    if (bytes % width || (order && (s->np % 512 || run % width))) continue;
     for (uint32_t threads : {128u, 256u}) {
       for (uint32_t remap : {0u, 8u}) {
         auto launch = [&]() {
-          return hlb::launch_copy_arrays(s->Q, bytes, bytes, width, src, dst, threads, remap,
-                                         s->stream);
+          return hlb::launch_copy_arrays(s->Q, bytes, bytes, run, width, src, dst, threads,
+                                         remap, s->stream);
         };
         if (launch()) return fail(HLB_E_DEVICE, "copy kernel launch failed");
         HIP_OR_FAIL(hipEventRecord(s->ev0, s->stream));
@@ -1697,6 +1708,7 @@ int hlb_copy_ceiling(hlb_solver* s, int32_t reps, hlb_copy_result* out) {
           best.threads = (int32_t)threads;
           best.xcd_remap = (int32_t)remap;
           best.width = width;
+          best.order = order;
         }
       }
     }
@@ -1715,13 +1727,14 @@ int hlb_halo_pack(hlb_solver* s, const int64_t* pairs, int64_t count, double* ho
     const int64_t site = pairs[2 * k], d = pairs[2 * k + 1];
     if (site < 0 || site >= s->n || d < 0 || d >= s->Q)
       return fail(HLB_E_INVALID, "halo pair out of range");
-    slots[k] = d * s->np + s->slot_of[s->internal[site]];
+    slots[k] = s->state_index(d, s->slot_of[s->internal[site]]);
   }
   DevBuf<int64_t> d_slots;
   DevBuf<double> d_out;
   HIP_OR_FAIL(d_slots.upload(slots));
   HIP_OR_FAIL(d_out.alloc(count));
-  if (hlb::launch_gather_pairs(count, s->np, d_slots.p, s->state[s->cur], s->p.storage,
+  if (hlb::launch_gather_pairs(count, s->np, s->Q, s->blockwise(), d_slots.p,
+                               s->state[s->cur], s->p.storage,
                                s->args.rho0, s->weights.p, d_out.p, s->stream))
     return fail(HLB_E_DEVICE, "halo pack kernel launch failed");
   HIP_OR_FAIL(hipMemcpyAsync(host_buffer, d_out.p, sizeof(double) * count,
@@ -1741,7 +1754,7 @@ int hlb_halo_unpack(hlb_solver* s, const int64_t* pairs, int64_t count,
     const int64_t site = pairs[2 * k], d = pairs[2 * k + 1];
     if (site < 0 || site >= s->n || d < 0 || d >= s->Q)
       return fail(HLB_E_INVALID, "halo pair out of range");
-    slots[k] = d * s->np + s->slot_of[s->internal[site]];
+    slots[k] = s->state_index(d, s->slot_of[s->internal[site]]);
   }
   {
     std::vector<int64_t> sorted(slots);
@@ -1753,7 +1766,8 @@ int hlb_halo_unpack(hlb_solver* s, const int64_t* pairs, int64_t count,
   DevBuf<double> d_in;
   HIP_OR_FAIL(d_slots.upload(slots));
   HIP_OR_FAIL(d_in.upload(std::vector<double>(host_buffer, host_buffer + count)));
-  if (hlb::launch_scatter_pairs(count, s->np, d_slots.p, d_in.p, s->p.storage, s->args.rho0,
+  if (hlb::launch_scatter_pairs(count, s->np, s->Q, s->blockwise(), d_slots.p, d_in.p,
+                                s->p.storage, s->args.rho0,
                                 s->weights.p, s->state[s->cur], s->stream))
     return fail(HLB_E_DEVICE, "halo unpack kernel launch failed");
   HIP_OR_FAIL(hipStreamSynchronize(s->stream));
@@ -1817,8 +1831,8 @@ int hlb_halo_plan_create(hlb_solver* src, hlb_solver* dst, hlb_halo_plan** out) 
       const int32_t t[3] = {c[0] + L.c[e][0], c[1] + L.c[e][1], c[2] + L.c[e][2]};
       const int64_t o_dst = find_site(dst, coord_key(t));
       if (o_dst < 0 || o_dst >= dst->n) continue;
-      pairs.push_back({e * src->np + src->slot_of[o_src],
-                       e * dst->np + dst->slot_of[dst->n + g]});
+      pairs.push_back({src->state_index(e, src->slot_of[o_src]),
+                       dst->state_index(e, dst->slot_of[dst->n + g])});
     }
   }
   std::sort(pairs.begin(), pairs.end());

@@ -336,7 +336,8 @@ class LBM:
         r = N.hlb_copy_result()
         N.check(N.lib().hlb_copy_ceiling(self._h, int(reps), C.byref(r)))
         return {"GBs": r.bytes_per_s / 1e9, "ms": r.ms, "bytes": r.bytes,
-                "threads": r.threads, "xcd_remap": r.xcd_remap, "width": r.width}
+                "threads": r.threads, "xcd_remap": r.xcd_remap, "width": r.width,
+                "order": "blocks" if r.order else "arrays"}
 
     # FieldData export/import
     def get_distributions(self) -> np.ndarray:

@@ -73,7 +73,8 @@ enum hlb_site_type { HLB_SITE_FLUID = 1, HLB_SITE_INLET = 2, HLB_SITE_OUTLET = 3
  * reductions stay binary64. */
 enum hlb_storage { HLB_STORE_FP64 = 0, HLB_STORE_FP32 = 1, HLB_STORE_FP16 = 2 };
 /* Layout of the distributions in HBM.  HLB_LAYOUT_BLOCKS gives every 8^3
- * block that holds fluid 512 slots (Morton block order, z fastest) and finds
+ * block that holds fluid 512 slots (Morton block order, z fastest), stores a
+ * block's directions together ([block][direction][slot]) and finds
  * neighbours by slot arithmetic; HLB_LAYOUT_PULL_TABLE packs the fluid sites
  * and reads a 32-bit pull index per site and direction;
  * HLB_LAYOUT_COMPACT_BLOCKS packs the fluid sites in the same order and finds
@@ -284,11 +285,13 @@ int hlb_velocity_change(hlb_solver* s, double reference_speed, double* out);
  * counterpart in the reference): the solver's Q state arrays copied from the
  * current state buffer into the other one (free between steps, so the state
  * is kept), reps times per variant, one word per thread and array with the
- * loads issued before streamed stores.  Variants: words of the storage's
- * element size (4 bytes at least) or 16 bytes, 128- or 256-thread
- * workgroups, the hardware workgroup order or runs of 8 workgroups per XCD.
- * Reports the fastest: read + write bytes per second, milliseconds per copy,
- * bytes per copy and its launch. */
+ * loads issued before streamed stores.  Variants: the words of one array per
+ * direction (order 0) or in the block layout's order (order 1: runs of 512
+ * slots per direction and block), words of the storage's element size (4
+ * bytes at least) or 16 bytes, 128- or 256-thread workgroups, the hardware
+ * workgroup order or runs of 8 workgroups per XCD.  Reports the fastest:
+ * read + write bytes per second, milliseconds per copy, bytes per copy and
+ * its launch. */
 typedef struct {
   double bytes_per_s;
   double ms;
@@ -296,7 +299,7 @@ typedef struct {
   int32_t threads;
   int32_t xcd_remap;
   int32_t width;
-  int32_t reserved;
+  int32_t order;
 } hlb_copy_result;
 int hlb_copy_ceiling(hlb_solver* s, int32_t reps, hlb_copy_result* out);
 
