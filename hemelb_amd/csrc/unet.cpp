@@ -104,7 +104,7 @@ int refresh_weights(hlbn_net* n) {
 int conv_layer(hlbn_net* n, int L, const uint16_t* x, uint16_t* y) {
   const Conv& c = n->conv[L];
   KERN(hlbn::conv_fwd(n->geo[c.level], x, c.ci, c.w16.p, n->params.p + c.boff, c.co, nullptr,
-                      y, c.co, 0, 1, n->stream));
+                      y, c.co, 0, 1, n->part.p, n->stream));
   return 0;
 }
 
@@ -121,7 +121,7 @@ int conv_layer_back(hlbn_net* n, int L, const uint16_t* x, const float* dpost, f
                          n->grads.p + c.boff, n->stream));
   if (dx)
     KERN(hlbn::conv_fwd(G, n->dpre16.p, c.co, c.wt16.p, nullptr, c.ci, dx, nullptr, c.ci, 0, 0,
-                        n->stream));
+                        n->part.p, n->stream));
   return 0;
 }
 
@@ -281,7 +281,10 @@ int hlbn_create(int32_t side, int32_t batch, int32_t in_channels, int32_t out_ch
     HIPN(c.wt16.alloc((size_t)27 * c.co * c.ci));
     const int64_t P = n->P[c.level];
     const int64_t ch = hlbn::wgrad_chunk(n->geo[c.level]);
-    max_part = std::max(max_part, (P + ch - 1) / ch * 27 * c.co * c.ci);
+    max_part = std::max(max_part, hlbn::wgrad_parts(n->geo[c.level], c.ci, c.co));
+    // K-slice workspaces of the forward and backward-data convolutions
+    max_part = std::max(max_part, hlbn::conv_fwd_ws(n->geo[c.level], c.ci, c.co));
+    max_part = std::max(max_part, hlbn::conv_fwd_ws(n->geo[c.level], c.co, c.ci));
     max_part = std::max(max_part, (P + ch - 1) / ch * c.co);
     max_pc = std::max(max_pc, P * c.co);
   }

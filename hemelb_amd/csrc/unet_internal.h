@@ -13,15 +13,20 @@ struct ConvGeom {
 
 // y[p][yoff + o] = (relu)(sum_t,c x[p + off_t][c] w[t][o][c] + bias[o]) into
 // a binary32 and/or bfloat16 tensor with yc channels.  ci, co multiples of 16.
+// ws: conv_fwd_ws(g, ci, co) floats of workspace for the K slices of coarse
+// levels (null when that is 0).
 int conv_fwd(const ConvGeom& g, const uint16_t* x, int ci, const uint16_t* w,
              const float* bias, int co, float* y32, uint16_t* y16, int yc, int yoff, int relu,
-             void* stream);
+             float* ws, void* stream);
+int64_t conv_fwd_ws(const ConvGeom& g, int ci, int co);
 // Position chunk of the backward-weight partial sums.
 int64_t wgrad_chunk(const ConvGeom& g);
 // dw[t][o][c] = sum_p dy[p][o] x[p + off_t][c]; part holds
-// ceil(P / wgrad_chunk) * 27 * co * ci floats.
+// wgrad_parts(g, ci, co) floats (partial sums over runs of positions, added
+// in a fixed order).
 int conv_wgrad(const ConvGeom& g, const uint16_t* x, int ci, const uint16_t* dy, int co,
                float* part, float* dw, void* stream);
+int64_t wgrad_parts(const ConvGeom& g, int ci, int co);
 // out[c] = sum_p a[p][c] (chunked, fixed order); part holds ceil(P/chunk)*C.
 int channel_sum(const float* a, int64_t P, int C, float* part, int64_t chunk, float* out,
                 void* stream);

@@ -11,6 +11,7 @@
 // t = 9*kd + 3*kh + kw for offsets (kd-1, kh-1, kw-1).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "unet_internal.h"
@@ -20,6 +21,7 @@ namespace {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) {
   return __uint_as_float((uint32_t)b << 16);
@@ -57,75 +59,139 @@ __device__ __forceinline__ void advance(const Grid3& g, Coord3& c, int step) {
 }
 
 // --------------------------------------------------------------------------
-// Forward 3x3x3 convolution (also backward-data).  One wave computes 16
-// consecutive positions x CT tiles of 16 output channels with
-// v_mfma_f32_16x16x16_bf16: A = input patches (16 positions x 16 channels of
-// one tap), B = weights (16 channels x 16 outputs).  Lane l supplies A row
-// l%16 and B column l%16, K elements 4*(l/16)..+3; it receives D rows
-// 4*(l/16)..+3 of column l%16.
+// Forward 3x3x3 convolution (also backward-data): an implicit GEMM on
+// gfx950's v_mfma_f32_16x16x32_bf16 with M = positions, N = output channels
+// and K = 27 taps x input channels.  One wave computes MT tiles of 16
+// consecutive positions x NT tiles of 16 output channels.  K advances in
+// steps of 32: two half-steps of 16 (tap, 16-channel group) pairs, tap-major,
+// the 28th tap of an odd count padded with zeros.  Lane l supplies row l%16
+// of A (its position) and column l%16 of B (its output channel) with the 8
+// K-values 8*(l/16)..+7, each one 16-byte load of 8 consecutive channels
+// (channels-last activations, [27][Cout][Cin] weights); it receives rows
+// 4*(l/16)..+3 of column l%16.  An A fragment serves NT MFMAs, a B fragment
+// MT.  Coarse levels, with few positions and long K, split K over
+// gridDim.y slices of `hpw` half-steps: each slice writes its raw sums to
+// ws[slice][position][co] and k_conv_reduce adds them in slice order.
 // --------------------------------------------------------------------------
-template <int CT>
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8_t load8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(p));
+}
+
+template <int MT, int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t* __restrict__ x,
                                                   int ci, const uint16_t* __restrict__ w,
                                                   const float* __restrict__ bias, int co,
-                                                  int tiles_per_wave,
                                                   float* __restrict__ y32,
                                                   uint16_t* __restrict__ y16, int yc,
-                                                  int yoff, int relu) {
+                                                  int yoff, int relu, int hpw,
+                                                  float* __restrict__ ws) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int ngroups = (co / 16 + tiles_per_wave - 1) / tiles_per_wave;
+  const int ngroups = (co / 16 + NT - 1) / NT;
   const int64_t pgroup = wave / ngroups;
-  const int tile0 = (int)(wave % ngroups) * tiles_per_wave;
+  const int tile0 = (int)(wave % ngroups) * NT;
   const int64_t P = g.P();
-  const int64_t p0 = pgroup * 16;
+  const int64_t p0 = pgroup * 16 * MT;
   if (p0 >= P) return;
-  const int row = lane & 15, kq = lane >> 4;
-  const int64_t p = p0 + row;
-  const bool pvalid = p < P;
-  float4_t acc[CT];
+  const int row = lane & 15, q = lane >> 4;
+  const int half = q >> 1, sub = q & 1;
+  int64_t pp[MT];
+  bool pv[MT];
+  Coord3 pc[MT];
 #pragma unroll
-  for (int j = 0; j < CT; ++j) acc[j] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const int ntile = min(CT, co / 16 - tile0);
-  // the lane's position decomposed once; each tap checks its neighbour
-  // against the volume (zero padding outside)
-  const Coord3 pc = pvalid ? coord_of(g, (int)p) : Coord3{0, 0, 0};
-  for (int t = 0; t < 27; ++t) {
-    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-    const bool inside = pvalid && (unsigned)(pc.d + dz) < (unsigned)g.D &&
-                        (unsigned)(pc.h + dy) < (unsigned)g.H &&
-                        (unsigned)(pc.w + dx) < (unsigned)g.W;
-    const int64_t q = inside ? p + ((int64_t)dz * g.H + dy) * g.W + dx : -1;
-    for (int c0 = 0; c0 < ci; c0 += 16) {
-      short4_t a = short4_t{0, 0, 0, 0};
-      if (q >= 0) a = *reinterpret_cast<const short4_t*>(x + q * ci + c0 + 4 * kq);
+  for (int m = 0; m < MT; ++m) {
+    pp[m] = p0 + 16 * m + row;
+    pv[m] = pp[m] < P;
+    pc[m] = pv[m] ? coord_of(g, (int)pp[m]) : Coord3{0, 0, 0};
+  }
+  f32x4_t acc[MT][NT];
 #pragma unroll
-      for (int j = 0; j < CT; ++j) {
-        if (j < ntile) {
-          const int oc = (tile0 + j) * 16 + row;
-          const short4_t b =
-              *reinterpret_cast<const short4_t*>(w + ((int64_t)t * co + oc) * ci + c0 + 4 * kq);
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
-        }
-      }
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[m][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int cpt = ci / 16;       // 16-channel groups per tap
+  const int split = blockIdx.y;
+  const int hbeg = split * hpw;
+  const int nhalf = min(27 * cpt, hbeg + hpw);  // half-steps of 16 K-values
+  const bf16x8_t zero = __builtin_bit_cast(bf16x8_t, u32x4_t{0u, 0u, 0u, 0u});
+  for (int h0 = hbeg; h0 < nhalf; h0 += 2) {
+    const int h = h0 + half;
+    const bool hv = h < nhalf;
+    const int t = hv ? h / cpt : 0;
+    const int c = (h - t * cpt) * 16 + 8 * sub;
+    const int dz = t / 9 - 1, dyy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const int64_t off = ((int64_t)dz * g.H + dyy) * g.W + dx;
+    bf16x8_t a[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const bool inside = hv && pv[m] && (unsigned)(pc[m].d + dz) < (unsigned)g.D &&
+                          (unsigned)(pc[m].h + dyy) < (unsigned)g.H &&
+                          (unsigned)(pc[m].w + dx) < (unsigned)g.W;
+      a[m] = inside ? load8(x + (pp[m] + off) * ci + c) : zero;
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int oc = (tile0 + j) * 16 + row;
+      const bf16x8_t b = (hv && oc < co) ? load8(w + ((int64_t)t * co + oc) * ci + c) : zero;
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][j], 0, 0, 0);
     }
   }
+  if (ws) {
 #pragma unroll
-  for (int j = 0; j < CT; ++j) {
-    if (j >= ntile) continue;
+    for (int j = 0; j < NT; ++j) {
+      const int oc = (tile0 + j) * 16 + row;
+      if (oc >= co) continue;
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t pos = p0 + 16 * m + 4 * q + r;
+          if (pos < P) ws[((int64_t)split * P + pos) * co + oc] = acc[m][j][r];
+        }
+    }
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
     const int oc = (tile0 + j) * 16 + row;
+    if (oc >= co) continue;
     const float bv = bias ? bias[oc] : 0.f;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t pp = p0 + 4 * kq + r;
-      if (pp >= P) continue;
-      float v = acc[j][r] + bv;
-      if (relu) v = v > 0.f ? v : 0.f;
-      const int64_t o = pp * yc + yoff + oc;
-      if (y32) y32[o] = v;
-      if (y16) y16[o] = f2bf(v);
-    }
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t pos = p0 + 16 * m + 4 * q + r;
+        if (pos >= P) continue;
+        float v = acc[m][j][r] + bv;
+        if (relu) v = v > 0.f ? v : 0.f;
+        const int64_t o = pos * yc + yoff + oc;
+        if (y32) y32[o] = v;
+        if (y16) y16[o] = f2bf(v);
+      }
   }
+}
+
+// y = (relu)(sum over the K slices of ws, in slice order, + bias).
+__global__ void k_conv_reduce(int64_t P, int co, int nsplit, const float* __restrict__ ws,
+                              const float* __restrict__ bias, int relu,
+                              float* __restrict__ y32, uint16_t* __restrict__ y16, int yc,
+                              int yoff) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * co) return;
+  const int oc = (int)(i % co);
+  const int64_t pos = i / co;
+  float v = 0.f;
+  for (int k = 0; k < nsplit; ++k) v += ws[(int64_t)k * P * co + i];
+  v += bias ? bias[oc] : 0.f;
+  if (relu) v = v > 0.f ? v : 0.f;
+  const int64_t o = pos * yc + yoff + oc;
+  if (y32) y32[o] = v;
+  if (y16) y16[o] = f2bf(v);
 }
 
 // --------------------------------------------------------------------------
@@ -202,6 +268,133 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Grid3 g, const uint16_t* __r
     float* out = part + ((int64_t)(k * 27 + t) * co) * ci;
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[(m0 * 16 + 4 * kq + r) * ci + ic] = acc[tt][r];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Backward-weight on v_mfma_f32_16x16x32_bf16 with the operands staged in
+// LDS: dW[t][co][ci] = sum_p dy[p][co] x[p + off_t][ci] is, per tap, a GEMM
+// with M = 16 output channels, N = 16 input channels and K = positions.  A
+// workgroup takes one (co, ci) tile pair and a run of `bpw` boxes of
+// BD x BH x BW = 256 positions (BW = 16 or 8).  Per box it stages
+//  * sA: dy of the box transposed to [co][position] (one 16-byte row
+//    segment per lane and K step), and
+//  * sB: x of the box and its one-voxel halo transposed to
+//    [dx][ci][d][h][w], three copies shifted by dx = -1, 0, +1, so that the
+//    eight positions a lane needs for any tap are one aligned 16-byte read;
+// then wave w accumulates taps w, w+4, ... over the box's 8 K steps of 32
+// positions (octets of 8 consecutive w).  Rows are padded by 8 elements
+// against LDS bank conflicts.  Each run of boxes writes one partial sum.
+// --------------------------------------------------------------------------
+template <int BW>
+struct WBox {
+  static constexpr int BD = 4, BH = 256 / (4 * BW);
+  static constexpr int HD = BD + 2, HH = BH + 2;
+  static constexpr int ROW_A = 256 + 8;             // [co][position] row
+  static constexpr int ROW_B = HD * HH * BW + 8;    // [ci][d][h][w] plane
+};
+
+template <int BW>
+__global__ __launch_bounds__(256) void k_conv_wgrad_box(Grid3 g, const uint16_t* __restrict__ x,
+                                                        int ci, const uint16_t* __restrict__ dy,
+                                                        int co, int bpw,
+                                                        float* __restrict__ part) {
+  using B = WBox<BW>;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[16 * B::ROW_A];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[3 * 16 * B::ROW_B];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int row = lane & 15, q = lane >> 4;
+  const int nct = co / 16, nit = ci / 16;
+  const int tile = blockIdx.x % (nct * nit);
+  const int chunk = blockIdx.x / (nct * nit);
+  const int co0 = (tile / nit) * 16, ci0 = (tile % nit) * 16;
+  const int nbd = g.D / B::BD, nbh = g.H / B::BH, nbw = g.W / BW;
+  const int nboxes = g.N * nbd * nbh * nbw;
+  f32x4_t acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) acc[k] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int b_end = min(nboxes, (chunk + 1) * bpw);
+  for (int b = chunk * bpw; b < b_end; ++b) {
+    const int bw0 = (b % nbw) * BW, bh0 = ((b / nbw) % nbh) * B::BH;
+    const int bd0 = ((b / (nbw * nbh)) % nbd) * B::BD, n = b / (nbw * nbh * nbd);
+    __syncthreads();  // the previous box's reads are done
+    {
+      // dy: thread = (octet o of 8 positions, co pair j)
+      const int o = tid >> 3, j = tid & 7;
+      const int od = o / (B::BH * BW / 8), oh = (o / (BW / 8)) % B::BH, ow = (o % (BW / 8)) * 8;
+      const int64_t p = (((int64_t)n * g.D + bd0 + od) * g.H + bh0 + oh) * g.W + bw0 + ow;
+      uint32_t v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        v[i] = *reinterpret_cast<const uint32_t*>(dy + (p + i) * co + co0 + 2 * j);
+      u32x4_t lo, hi;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        lo[i] = (v[2 * i] & 0xFFFFu) | (v[2 * i + 1] << 16);
+        hi[i] = (v[2 * i] >> 16) | (v[2 * i + 1] & 0xFFFF0000u);
+      }
+      *reinterpret_cast<u32x4_t*>(&sA[(2 * j) * B::ROW_A + o * 8]) = lo;
+      *reinterpret_cast<u32x4_t*>(&sA[(2 * j + 1) * B::ROW_A + o * 8]) = hi;
+    }
+    // x halo: task = (halo row (hd, hh), w octet, ci pair)
+    constexpr int ntask = B::HD * B::HH * (BW / 8) * 8;
+    for (int task = tid; task < ntask; task += 256) {
+      const int j = task & 7, rest = task >> 3;
+      const int wo = rest % (BW / 8), r2 = rest / (BW / 8);
+      const int hh = r2 % B::HH, hd = r2 / B::HH;
+      const int d = bd0 + hd - 1, h = bh0 + hh - 1;
+      const bool rowin = (unsigned)d < (unsigned)g.D && (unsigned)h < (unsigned)g.H;
+      const int64_t rbase = (((int64_t)n * g.D + d) * g.H + h) * g.W;
+      uint32_t v[10];
+#pragma unroll
+      for (int i = 0; i < 10; ++i) {
+        const int w = bw0 + wo * 8 + i - 1;
+        v[i] = (rowin && (unsigned)w < (unsigned)g.W)
+                   ? *reinterpret_cast<const uint32_t*>(x + (rbase + w) * ci + ci0 + 2 * j)
+                   : 0u;
+      }
+#pragma unroll
+      for (int sh = 0; sh < 3; ++sh) {
+        u32x4_t lo, hi;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t a = v[sh + 2 * i], c = v[sh + 2 * i + 1];
+          lo[i] = (a & 0xFFFFu) | (c << 16);
+          hi[i] = (a >> 16) | (c & 0xFFFF0000u);
+        }
+        const int base = ((hd * B::HH + hh) * BW) + wo * 8;
+        *reinterpret_cast<u32x4_t*>(&sB[(sh * 16 + 2 * j) * B::ROW_B + base]) = lo;
+        *reinterpret_cast<u32x4_t*>(&sB[(sh * 16 + 2 * j + 1) * B::ROW_B + base]) = hi;
+      }
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int ks = 0; ks < 8; ++ks) {
+      const int o = 4 * ks + q;
+      const int od = o / (B::BH * BW / 8), oh = (o / (BW / 8)) % B::BH, ow = (o % (BW / 8)) * 8;
+      const bf16x8_t a = __builtin_bit_cast(
+          bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sA[row * B::ROW_A + o * 8]));
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const int t = wv + 4 * k;
+        if (t < 27) {
+          const int dz = t / 9, dyy = (t / 3) % 3, dx = t % 3;  // offsets + 1
+          const bf16x8_t bb = __builtin_bit_cast(
+              bf16x8_t, *reinterpret_cast<const u32x4_t*>(
+                            &sB[(dx * 16 + row) * B::ROW_B + ((od + dz) * B::HH + oh + dyy) * BW +
+                                ow]));
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[k], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const int t = wv + 4 * k;
+    if (t >= 27) continue;
+    float* out = part + ((int64_t)chunk * 27 + t) * co * ci;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[(int64_t)(co0 + 4 * q + r) * ci + ci0 + row] = acc[k][r];
   }
 }
 
@@ -463,28 +656,72 @@ void sum_chunks(const float* part, int64_t nchunks, int64_t n, float* out, hipSt
 
 }  // namespace
 
-int conv_fwd(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* w,
-             const float* bias, int co, float* y32, uint16_t* y16, int yc, int yoff, int relu,
-             void* stream) {
-  const Grid3 g{cg.N, cg.D, cg.H, cg.W};
+namespace {
+// Launch plan of a forward convolution: NT output-channel tiles per wave
+// (an A fragment feeds NT MFMAs), MT position tiles (a B fragment feeds MT)
+// and the number of K slices.  Fine levels keep MT = 4 or 2; coarse levels,
+// with few positions, first take fewer position tiles per wave and then
+// split K so that the grid has about four waves per SIMD and each wave a
+// short chain of K steps.
+struct FwdPlan {
+  int mt, nt, nsplit, hpw;
+  int64_t waves;
+};
+FwdPlan fwd_plan(const ConvGeom& cg, int ci, int co) {
   const int64_t P = (int64_t)cg.N * cg.D * cg.H * cg.W;
   const int tiles = co / 16;
-  int tpw = tiles >= 8 ? 8 : tiles;  // tiles per wave
-  // Coarse levels have few positions: fewer tiles per wave, so that the grid
-  // still has about four waves per SIMD (the input patches are then loaded
-  // once per tile group).
-  while (tpw > 1 && ((P + 15) / 16) * ((tiles + tpw - 1) / tpw) < 4096) tpw = (tpw + 1) / 2;
-  const int ngroups = (tiles + tpw - 1) / tpw;
-  const int64_t waves = ((P + 15) / 16) * ngroups;
-  const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
-  hipStream_t s = (hipStream_t)stream;
-  switch (tpw) {
-    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, block, 0, s, g, x, ci, w, bias, co, tpw, y32, y16, yc, yoff, relu); break;
-    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, block, 0, s, g, x, ci, w, bias, co, tpw, y32, y16, yc, yoff, relu); break;
-    case 3: hipLaunchKernelGGL(k_conv_fwd<4>, grid, block, 0, s, g, x, ci, w, bias, co, tpw, y32, y16, yc, yoff, relu); break;
-    case 4: hipLaunchKernelGGL(k_conv_fwd<4>, grid, block, 0, s, g, x, ci, w, bias, co, tpw, y32, y16, yc, yoff, relu); break;
-    default: hipLaunchKernelGGL(k_conv_fwd<8>, grid, block, 0, s, g, x, ci, w, bias, co, tpw, y32, y16, yc, yoff, relu); break;
+  FwdPlan f;
+  f.nt = tiles >= 8 ? 8 : (tiles >= 3 ? 4 : tiles);
+  f.mt = f.nt >= 8 ? 2 : 4;
+  const int ngroups = (tiles + f.nt - 1) / f.nt;
+  constexpr int64_t kTarget = 4096;
+  while (f.mt > 1 && ((P + 16 * f.mt - 1) / (16 * f.mt)) * ngroups < kTarget) f.mt /= 2;
+  f.waves = ((P + 16 * f.mt - 1) / (16 * f.mt)) * ngroups;
+  const int nhalf = 27 * (ci / 16);
+  const int ksteps = (nhalf + 1) / 2;
+  int ns = (int)std::min<int64_t>(ksteps, (kTarget + f.waves - 1) / f.waves);
+  int kps = (ksteps + ns - 1) / ns;  // K steps per slice
+  f.hpw = 2 * kps;
+  f.nsplit = (nhalf + f.hpw - 1) / f.hpw;
+  if (f.nsplit <= 1) {
+    f.nsplit = 1;
+    f.hpw = nhalf + 1;
   }
+  return f;
+}
+}  // namespace
+
+int64_t conv_fwd_ws(const ConvGeom& cg, int ci, int co) {
+  const FwdPlan f = fwd_plan(cg, ci, co);
+  return f.nsplit > 1 ? (int64_t)f.nsplit * cg.N * cg.D * cg.H * cg.W * co : 0;
+}
+
+int conv_fwd(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* w,
+             const float* bias, int co, float* y32, uint16_t* y16, int yc, int yoff, int relu,
+             float* ws, void* stream) {
+  const Grid3 g{cg.N, cg.D, cg.H, cg.W};
+  const int64_t P = (int64_t)cg.N * cg.D * cg.H * cg.W;
+  const FwdPlan f = fwd_plan(cg, ci, co);
+  if (f.nsplit > 1 && !ws) return (int)hipErrorInvalidValue;
+  float* slices = f.nsplit > 1 ? ws : nullptr;
+  const dim3 grid((unsigned)((f.waves + 3) / 4), (unsigned)f.nsplit), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  bool launched = false;
+#define HLBN_FWD(M_, N_)                                                                   \
+  if (!launched && f.mt == M_ && f.nt == N_) {                                             \
+    hipLaunchKernelGGL((k_conv_fwd<M_, N_>), grid, block, 0, s, g, x, ci, w, bias, co, y32,  \
+                       y16, yc, yoff, relu, f.hpw, slices);                                \
+    launched = true;                                                                       \
+  }
+  HLBN_FWD(4, 1) HLBN_FWD(2, 1) HLBN_FWD(1, 1)
+  HLBN_FWD(4, 2) HLBN_FWD(2, 2) HLBN_FWD(1, 2)
+  HLBN_FWD(4, 4) HLBN_FWD(2, 4) HLBN_FWD(1, 4)
+  HLBN_FWD(2, 8) HLBN_FWD(1, 8)
+#undef HLBN_FWD
+  if (!launched) return (int)hipErrorInvalidValue;
+  if (slices)
+    hipLaunchKernelGGL(k_conv_reduce, blocks_for(P * co), dim3(256), 0, s, P, co, f.nsplit,
+                       slices, bias, relu, y32, y16, yc, yoff);
   return (int)hipGetLastError();
 }
 
@@ -497,18 +734,62 @@ int64_t wgrad_chunk(const ConvGeom& cg) {
   return chunk < 256 ? 256 : chunk;
 }
 
+namespace {
+// Box width of the LDS-staged weight gradient for this level (0: the level
+// does not tile into 4 x BH x BW boxes and takes the direct kernel).
+int wgrad_box_width(const ConvGeom& cg) {
+  if (cg.D % 4) return 0;
+  if (cg.W % 16 == 0 && cg.H % 4 == 0) return 16;
+  if (cg.W % 8 == 0 && cg.H % 8 == 0) return 8;
+  return 0;
+}
+// Boxes per workgroup: at least about 1024 workgroups over the tile pairs.
+int64_t wgrad_boxes_per_group(const ConvGeom& cg, int ci, int co) {
+  const int64_t nboxes = (int64_t)cg.N * cg.D * cg.H * cg.W / 256;
+  const int64_t tiles = (int64_t)(co / 16) * (ci / 16);
+  return std::max<int64_t>(1, nboxes * tiles / 1024);
+}
+}  // namespace
+
+int64_t wgrad_parts(const ConvGeom& cg, int ci, int co) {
+  const int64_t P = (int64_t)cg.N * cg.D * cg.H * cg.W;
+  int64_t nchunks;
+  if (wgrad_box_width(cg)) {
+    const int64_t bpw = wgrad_boxes_per_group(cg, ci, co);
+    nchunks = (P / 256 + bpw - 1) / bpw;
+  } else {
+    const int64_t chunk = wgrad_chunk(cg);
+    nchunks = (P + chunk - 1) / chunk;
+  }
+  return nchunks * 27 * co * ci;
+}
+
 int conv_wgrad(const ConvGeom& cg, const uint16_t* x, int ci, const uint16_t* dy, int co,
                float* part, float* dw, void* stream) {
   const Grid3 g{cg.N, cg.D, cg.H, cg.W};
   const int64_t P = g.P();
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = (int64_t)27 * co * ci;
+  const int bw = wgrad_box_width(cg);
+  if (bw) {
+    const int64_t bpw = wgrad_boxes_per_group(cg, ci, co);
+    const int64_t nchunks = (P / 256 + bpw - 1) / bpw;
+    const dim3 grid((unsigned)(nchunks * (co / 16) * (ci / 16))), block(256);
+    if (bw == 16)
+      hipLaunchKernelGGL(k_conv_wgrad_box<16>, grid, block, 0, s, g, x, ci, dy, co, (int)bpw,
+                         part);
+    else
+      hipLaunchKernelGGL(k_conv_wgrad_box<8>, grid, block, 0, s, g, x, ci, dy, co, (int)bpw,
+                         part);
+    sum_chunks(part, nchunks, n, dw, s);
+    return (int)hipGetLastError();
+  }
   const int64_t chunk = wgrad_chunk(cg);
   const int64_t nchunks = (P + chunk - 1) / chunk;
   constexpr int TG = 9;
   const int64_t waves = nchunks * (27 / TG) * (co / 16) * (ci / 16);
-  hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL((k_conv_wgrad<TG>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g,
                      x, ci, dy, co, chunk, part);
-  const int64_t n = (int64_t)27 * co * ci;
   sum_chunks(part, nchunks, n, dw, s);
   return (int)hipGetLastError();
 }
