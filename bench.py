@@ -185,7 +185,7 @@ def ensemble_bench(args) -> int:
                    "fluid_sites": n, "sites_per_vessel": [min(sizes), max(sizes)],
                    "storage": storage, "launch": {k: best[k] for k in
                                                   ("threads", "planes", "groups", "xcd_remap",
-                                                   "split")},
+                                                   "split", "wide")},
                    "setup_s": round(setup_s, 2), "bytes_alg_per_site": b_alg},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
@@ -211,6 +211,9 @@ def main() -> int:
     ap.add_argument("--split", type=int, default=-1,
                     help="planes with and without boundary sites in two launches (1) or "
                          "one (0); -1: the solver's default")
+    ap.add_argument("--wide", type=int, default=-1,
+                    help="1: 64-bit element addressing in the step even below 4 GiB of "
+                         "state; -1: the solver's default")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="oracle steps for the CPU baseline (0: about 10 s of them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -254,16 +257,23 @@ def main() -> int:
     setup_s = time.time() - t0
     launch = None
     if args.layout != "pull_table":
-        if args.threads or args.xcd_remap >= 0 or args.split >= 0:
+        if args.threads or args.xcd_remap >= 0 or args.split >= 0 or args.wide >= 0:
             sh = lbm.launch_shape
             lbm.set_launch_shape(threads=args.threads or sh["threads"], planes=sh["planes"],
                                  groups=sh["groups"],
                                  xcd_remap=sh["xcd_remap"] if args.xcd_remap < 0
                                  else args.xcd_remap,
-                                 split=sh["split"] if args.split < 0 else args.split)
+                                 split=sh["split"] if args.split < 0 else args.split,
+                                 wide=sh["wide"] if args.wide < 0 else args.wide)
         launch = lbm.launch_shape
     log(f"[bench] solver set-up {setup_s:.1f}s, device memory "
         f"{lbm.device_bytes() / 2**30:.2f} GiB")
+    cc = None
+    if not args.no_copy_ceiling:
+        # the library's copy of the same Q state arrays (fastest of its
+        # variants), measured first: its ~0.2 s of device work also brings
+        # the GPU to the clocks of a running simulation before the timed steps
+        cc = lbm.copy_ceiling(reps=20)
     lbm.step(args.warmup)
     lbm.synchronize()
     t_wall = time.perf_counter()
@@ -292,10 +302,7 @@ def main() -> int:
             pass
     log(f"[bench] {args.steps} steps: {ms:.2f} ms on the solver stream "
         f"({wall:.3f}s wall), {mlups:.1f} MLUPS, {achieved:.0f} GB/s algorithmic")
-    if not args.no_copy_ceiling:
-        # the library's copy of the same Q state arrays, with the step's
-        # workgroup sizes and XCD mapping (fastest of its variants)
-        cc = lbm.copy_ceiling(reps=20)
+    if cc:
         roof["copy_ceiling"] = round(cc["GBs"], 1)
         roof["frac_of_copy"] = round(achieved / cc["GBs"], 4)
         roof["copy_kernel"] = {k: cc[k] for k in ("threads", "xcd_remap", "width", "order",
@@ -314,8 +321,10 @@ def main() -> int:
             other = work.solver(storage=st, layout=args.layout)
             if launch and (args.threads or args.xcd_remap >= 0 or args.split >= 0):
                 other.set_launch_shape(**launch)
-            f_steps = args.form_steps or args.steps
-            f_ms, f_mlups = timed_rate(other, n, f_steps, args.warmup)
+            # more steps than the headline's, after a longer warm-up: the
+            # forms' rates are steady-state figures
+            f_steps = args.form_steps or max(args.steps, 200)
+            f_ms, f_mlups = timed_rate(other, n, f_steps, max(args.warmup, 50))
             f_ach = 2 * 19 * STORE_BYTES[st] * n / (f_ms * 1e-3) / 1e9
             forms[st] = {"value": round(f_mlups, 2), "unit": "MLUPS",
                          "ms_per_step": round(f_ms, 5), "steps": f_steps,

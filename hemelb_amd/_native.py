@@ -64,7 +64,7 @@ class hlb_instance_monitor(C.Structure):
 
 class hlb_launch_shape(C.Structure):
     _fields_ = [("threads", C.c_int32), ("planes", C.c_int32), ("groups", C.c_int32),
-                ("xcd_remap", C.c_int32), ("split", C.c_int32)]
+                ("xcd_remap", C.c_int32), ("split", C.c_int32), ("wide", C.c_int32)]
 
 
 class hlb_copy_result(C.Structure):

@@ -107,6 +107,10 @@ struct StepArgs {
   // (FP64 7 %, FP32 13 %, FP16 1.5 %), so one launch is the default and the
   // split is a tuner candidate.
   uint32_t split = 0;
+  // Block layout: 1 addresses the state with 64-bit element indices even
+  // where 32-bit byte offsets reach every element (below 4 GiB per state
+  // buffer, where the step uses them by default).
+  uint32_t wide = 0;
 };
 
 // Launches; every function returns 0 or a HIP error code.

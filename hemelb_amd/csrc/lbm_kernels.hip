@@ -266,10 +266,140 @@ __host__ __device__ constexpr double mrt_diag(int k) {
   return acc;
 }
 
+// Sums over the binary32-rounded weights (exact, in binary64): sum_i w_i and
+// sum_i w_i c_ix^2 (the same for y and z on both lattices).
+template <int Q>
+__host__ __device__ constexpr double rounded_weight_sum() {
+  double s = 0.0;
+  for (int i = 0; i < Q; ++i) s += (double)(float)Lat<Q>::w[i];
+  return s;
+}
+template <int Q>
+__host__ __device__ constexpr double rounded_weight_c2() {
+  double s = 0.0;
+  for (int i = 0; i < Q; ++i) s += (double)(float)Lat<Q>::w[i] * Lat<Q>::c[i][0] * Lat<Q>::c[i][0];
+  return s;
+}
+// sum_i fl(3 w_i) c_ix^2 with the binary32 coefficients of the linear
+// equilibrium term
+template <int Q>
+__host__ __device__ constexpr double rounded_3w_c2() {
+  double s = 0.0;
+  for (int i = 0; i < Q; ++i)
+    s += (double)(float)(3.0 * Lat<Q>::w[i]) * Lat<Q>::c[i][0] * Lat<Q>::c[i][0];
+  return s;
+}
+template <int Q>
+__host__ __device__ constexpr bool pairs_opposite() {
+  for (int i = 1; i < Q; i += 2)
+    for (int k = 0; k < 3; ++k)
+      if (Lat<Q>::c[i + 1][k] != -Lat<Q>::c[i][k]) return false;
+  return true;
+}
+
+// LBGK of the shifted storage forms in binary32 (these follow no reference
+// rounding).  Directions come in opposite pairs (d, d+1), so the moments are
+// sums of pair sums and differences, and a pair's equilibria are A +- B with
+// A = w (q + 9/2 e^2 / rho) and B = 3 w e, e = c_d . m.
+// Conservation: with binary32 coefficients, sum_i w_i = W, sum_i w_i c_i c_i
+// = G I and sum_i fl(3 w_i) c_i c_i = K I differ from 1, 1/3 and 1 by 1e-8 to
+// 3e-8, so the equilibrium's mass and momentum are W q + (9/2) G m^2/rho and
+// K m instead of drho and m.  Relaxing towards it would act as a body force
+// of ~1e-8 m per step (a 2e-5 to 3e-5 velocity bias on the vessel tree).
+// Two corrections of f_neq before the relaxation, both applied to f_neq,
+// whose magnitude keeps them above the rounding of binary32:
+// * EXACT (binary32 storage): the mass and momentum of the computed f_neq
+//   are removed, n_d -= w_d (s0 + 3 c_d . s) with s0, s its sums; this also
+//   removes the rounding noise of each collision (velocity 1e-7 from
+//   binary64 on the tree after 10000 steps);
+// * otherwise (binary16 storage, whose own rounding is 1e-3): only the
+//   systematic amounts above, the mass one on the rest direction and the
+//   momentum one split over the axis pairs (1e-6 in binary32 arithmetic),
+//   at a fraction of the projection's instructions.
+template <int Q, typename T, bool EXACT>
+__device__ __forceinline__ Hydro<Q, T> collide_lbgk_shifted(T (&f)[Q], const StepArgs& a) {
+  using L = Lat<Q>;
+  static_assert(pairs_opposite<Q>(), "directions d and d+1 must be opposite");
+  constexpr double W = rounded_weight_sum<Q>(), G = rounded_weight_c2<Q>();
+  constexpr double K = rounded_3w_c2<Q>();
+  T r = f[0], mx = T(0), my = T(0), mz = T(0);
+  sfor<0, (Q - 1) / 2>([&](auto p) {
+    constexpr int i = 2 * decltype(p)::value + 1;
+    const T dd = f[i] - f[i + 1];
+    r = r + (f[i] + f[i + 1]);
+    mx = addc<L::c[i][0]>(mx, dd);
+    my = addc<L::c[i][1]>(my, dd);
+    mz = addc<L::c[i][2]>(mz, dd);
+  });
+  Hydro<Q, T> h;
+  h.rho = T(a.rho0) + r;
+  h.mx = mx;
+  h.my = my;
+  h.mz = mz;
+  const T inv = recip(h.rho);
+  const T m2 = mad(mz, mz, mad(my, my, mx * mx));
+  const T q = mad(T(-1.5) * m2, inv, r);
+  const T nine = T(4.5) * inv;
+  const T om = T(a.omega_f);
+  T n[Q];  // f_neq
+  n[0] = f[0] - T(L::w[0]) * q;
+  sfor<0, (Q - 1) / 2>([&](auto p) {
+    constexpr int i = 2 * decltype(p)::value + 1;
+    const T e = dotc<L::c[i][0], L::c[i][1], L::c[i][2]>(mx, my, mz);
+    const T A = mad(T(L::w[i]) * nine, e * e, T(L::w[i]) * q);
+    const T B = T(3.0 * L::w[i]) * e;
+    n[i] = (f[i] - A) - B;
+    n[i + 1] = (f[i + 1] - A) + B;
+  });
+  if constexpr (EXACT) {
+    T s0 = n[0], sx = T(0), sy = T(0), sz = T(0);
+    sfor<0, (Q - 1) / 2>([&](auto p) {
+      constexpr int i = 2 * decltype(p)::value + 1;
+      const T dd = n[i] - n[i + 1];
+      s0 = s0 + (n[i] + n[i + 1]);
+      sx = addc<L::c[i][0]>(sx, dd);
+      sy = addc<L::c[i][1]>(sy, dd);
+      sz = addc<L::c[i][2]>(sz, dd);
+    });
+    sx = T(3) * sx;
+    sy = T(3) * sy;
+    sz = T(3) * sz;
+    n[0] = mad(T(-L::w[0]), s0, n[0]);
+    sfor<0, (Q - 1) / 2>([&](auto p) {
+      constexpr int i = 2 * decltype(p)::value + 1;
+      const T e = dotc<L::c[i][0], L::c[i][1], L::c[i][2]>(sx, sy, sz);
+      n[i] = mad(T(-L::w[i]), s0 + e, n[i]);
+      n[i + 1] = mad(T(-L::w[i + 1]), s0 - e, n[i + 1]);
+    });
+  } else {
+    // systematic mass of f_neq: drho - (W q + (9/2) G m^2/rho)
+    n[0] = n[0] - mad(T(1.0 - W), r, T(1.5 * W - 4.5 * G) * (m2 * inv));
+    // systematic momentum of f_neq: (1 - K) m, half on each axis direction
+    constexpr double kHalf = 0.5 * (1.0 - K);
+    sfor<0, (Q - 1) / 2>([&](auto p) {
+      constexpr int i = 2 * decltype(p)::value + 1;
+      constexpr int cx = L::c[i][0], cy = L::c[i][1], cz = L::c[i][2];
+      if constexpr (cx * cx + cy * cy + cz * cz == 1) {
+        const T k = T(kHalf) * (cx != 0 ? mx : cy != 0 ? my : mz) * T(cx + cy + cz);
+        n[i] = n[i] - k;
+        n[i + 1] = n[i + 1] + k;
+      }
+    });
+  }
+  sfor<0, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    f[d] = mad(om, n[d], f[d]);
+  });
+  return h;
+}
+
 // Collide in place: f (post-stream, possibly shifted) -> f* ; returns the
 // moments the streamers of the next step need.
-template <int Q, int KERNEL, bool SHIFT, typename T>
+template <int Q, int KERNEL, typename S, typename T = typename Store<S>::T>
 __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
+  constexpr bool SHIFT = kShifted<S>;
+  if constexpr (SHIFT && KERNEL == 0)
+    return collide_lbgk_shifted<Q, T, std::is_same<S, float>::value>(f, a);
   using L = Lat<Q>;
   Hydro<Q, T> h;
   T r;
@@ -283,31 +413,6 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
       constexpr int d = decltype(di)::value;
       fneq[d] = f[d] - fe.template eq<Q, d>();
     });
-    if constexpr (KERNEL == 0) {
-      // In binary32 the rounded weights give sum_i w_i c_i c_i = Cs2 (1 + 7e-9)
-      // I, so the equilibrium carries momentum m (1 + 7e-9): relaxing every
-      // direction towards it (LBGK) would add a body force of ~1e-8 m per
-      // step, a 1e-5 relative velocity bias in slow vessels.  Remove the mass
-      // and momentum of f_neq so that the collision conserves both.  (MRT
-      // never relaxes the conserved moments.)
-      using L = Lat<Q>;
-      T s0 = T(0), sx = T(0), sy = T(0), sz = T(0);
-      sfor<0, Q>([&](auto di) {
-        constexpr int d = decltype(di)::value;
-        s0 = s0 + fneq[d];
-        sx = addc<L::c[d][0]>(sx, fneq[d]);
-        sy = addc<L::c[d][1]>(sy, fneq[d]);
-        sz = addc<L::c[d][2]>(sz, fneq[d]);
-      });
-      sx = T(3) * sx;
-      sy = T(3) * sy;
-      sz = T(3) * sz;
-      sfor<0, Q>([&](auto di) {
-        constexpr int d = decltype(di)::value;
-        const T e = dotc<L::c[d][0], L::c[d][1], L::c[d][2]>(sx, sy, sz);
-        fneq[d] = mad(T(-L::w[d]), s0 + e, fneq[d]);
-      });
-    }
   } else {
     h.rho = r;
     FeqCoef<T> fe(r, h.mx, h.my, h.mz);
@@ -603,9 +708,113 @@ struct CompactBlockLayout {
   }
 };
 
+// The block layout addressed by 32-bit byte offsets from the state buffer's
+// base, for states of every direction below 4 GiB.  A load or store is then
+// the kernel argument's 64-bit base (a scalar register pair) plus one 32-bit
+// lane offset, which the global instructions add themselves: the wave-uniform
+// parts (block, direction, plane, and the neighbour blocks' offsets) are
+// scalar arithmetic, and a gather costs a select and an add per lane instead
+// of 64-bit multiply-adds.  Same elements, same order as BlockLayout.
+struct BlockLayout32 : BlockLayout {
+  // byte offset of slot `lane` of direction d in this wave's plane
+  template <int Q, typename P>
+  static __device__ __forceinline__ uint32_t own_off(const SiteRef& r, uint32_t d) {
+    constexpr uint32_t ES = sizeof(P);
+    return (((r.w >> 3) * (uint32_t)Q + d) * 512u + ((r.w & 7u) << 6)) * ES + r.lane * ES;
+  }
+  template <int Q, typename P>
+  static __device__ __forceinline__ P* own_ptr(const SiteRef& r, P* base, int d) {
+    constexpr uint32_t ES = sizeof(P);
+    // wave-uniform part first (scalar), then the lane's offset
+    const uint32_t uni = (((r.w >> 3) * (uint32_t)Q + (uint32_t)d) * 512u + ((r.w & 7u) << 6)) * ES;
+    using C = std::conditional_t<std::is_const<P>::value, const char, char>;
+    return (P*)(((C*)base + uni) + r.lane * ES);
+  }
+  template <int Q, int E, typename P>
+  static __device__ __forceinline__ P* src_ptr(const SiteRef& r, P* base) {
+    constexpr int CX = Lat<Q>::c[E][0], CY = Lat<Q>::c[E][1], CZ = Lat<Q>::c[E][2];
+    constexpr uint32_t ES = sizeof(P), BS = (uint32_t)Q * 512u * ES;
+    const int lx = (int)(r.w & 7u);
+    const int ly = (int)(r.lane >> 3), lz = (int)(r.lane & 7u);
+    const bool bx = CX != 0 && (CX > 0 ? lx == 0 : lx == 7);  // wave-uniform
+    const bool by = CY != 0 && (CY > 0 ? ly == 0 : ly == 7);
+    const bool bz = CZ != 0 && (CZ > 0 ? lz == 0 : lz == 7);
+    constexpr int i000 = 13, i100 = 13 - 9 * CX, i010 = 13 - 3 * CY, i001 = 13 - CZ;
+    constexpr int i110 = 13 - 9 * CX - 3 * CY, i101 = 13 - 9 * CX - CZ;
+    constexpr int i011 = 13 - 3 * CY - CZ, i111 = 13 - 9 * CX - 3 * CY - CZ;
+    // wave-uniform: direction E's array in each candidate block, source plane
+    const uint32_t uni = ((uint32_t)E * 512u + ((uint32_t)((lx - CX) & 7) << 6)) * ES;
+    // (readfirstlane keeps these products scalar: without it the compiler
+    // sinks the multiply below the per-lane select, into a vector multiply)
+    const auto blk = [&](int32_t t) {
+      return __builtin_amdgcn_readfirstlane((uint32_t)t * BS + uni);
+    };
+    const uint32_t n00 = blk(bx ? r.tb[i100] : r.tb[i000]);
+    const uint32_t n10 = blk(bx ? r.tb[i110] : r.tb[i010]);
+    const uint32_t n01 = blk(bx ? r.tb[i101] : r.tb[i001]);
+    const uint32_t n11 = blk(bx ? r.tb[i111] : r.tb[i011]);
+    const uint32_t nb = by ? (bz ? n11 : n10) : (bz ? n01 : n00);
+    const uint32_t yz = (uint32_t)((((ly - CY) & 7) << 3) | ((lz - CZ) & 7)) * ES;
+    constexpr int DP = inv_dir(E);
+    // a cut link reads the site's own slot of the inverse direction
+    // (both offsets are formed first so that the choice is a select, not a
+    // divergent branch)
+    const uint32_t own = own_off<Q, P>(r, DP), src = nb + yz;
+    const uint32_t off = ((r.cut >> (DP - 1)) & 1u) ? own : src;
+    using C = std::conditional_t<std::is_const<P>::value, const char, char>;
+    return (P*)((C*)base + off);
+  }
+  // Streamed store of slot d of the lane's site: a buffer store whose scalar
+  // offset carries the wave-uniform part and whose vector offset is the
+  // lane's (the same register for every direction), with the non-temporal
+  // bit, as the 64-bit layouts' non-temporal stores.
+  template <int Q, typename S>
+  static __device__ __forceinline__ void store(const DevDomain& D, const SiteRef& r, S* base,
+                                               int d, S v) {
+    constexpr uint32_t ES = sizeof(S);
+    const uint32_t uni = (((r.w >> 3) * (uint32_t)Q + (uint32_t)d) * 512u + ((r.w & 7u) << 6)) * ES;
+    const uint32_t bytes = (uint32_t)D.np * (uint32_t)Q * ES;  // below 4 GiB
+    // word 3 of a raw buffer resource on gfx9 parts: 32-bit data format
+    const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)base, 0, (int)bytes, 0x00020000);
+    constexpr int kNonTemporal = 2;
+    const int voff = (int)(r.lane * ES);
+    if constexpr (ES == 2)
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), rs, voff,
+                                            (int)uni, kNonTemporal);
+    else if constexpr (ES == 4)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), rs, voff,
+                                            (int)uni, kNonTemporal);
+    else {
+      using U2 = unsigned int __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, voff, (int)uni,
+                                            kNonTemporal);
+    }
+  }
+};
+
 template <class Lay>
 constexpr bool kWaveLayout = std::is_same<Lay, BlockLayout>::value ||
+                             std::is_same<Lay, BlockLayout32>::value ||
                              std::is_same<Lay, CompactBlockLayout>::value;
+
+// Addresses of state elements: slot d of the lane's own site, and the value
+// streaming into slot E of it (its source, or its own inverse slot across a
+// cut link).  Element indices for the 64-bit layouts, byte offsets for
+// BlockLayout32.
+template <int Q, class Lay, typename P>
+__device__ __forceinline__ P* own_ptr(const DevDomain& D, const SiteRef& r, P* base, int d) {
+  if constexpr (std::is_same<Lay, BlockLayout32>::value)
+    return Lay::template own_ptr<Q>(r, base, d);
+  else
+    return base + Lay::template at<Q>(D, d, r.x);
+}
+template <int Q, int E, class Lay, typename P>
+__device__ __forceinline__ P* src_ptr(const DevDomain& D, const SiteRef& r, P* base) {
+  if constexpr (std::is_same<Lay, BlockLayout32>::value)
+    return Lay::template src_ptr<Q, E>(r, base);
+  else
+    return base + Lay::template index<Q, E>(D, r);
+}
 
 // ---------------------------------------------------------------------------
 // Streaming (pull) with boundary links
@@ -668,11 +877,10 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
                                        const S* __restrict__ fin, const SiteRef& r,
                                        T (&f)[Q]) {
   using St = Store<S>;
-  const int64_t x = r.x;
-  f[0] = St::ld(fin + Lay::template at<Q>(D, 0, x));
+  f[0] = St::ld(own_ptr<Q, Lay>(D, r, fin, 0));
   sfor<1, Q>([&](auto ei) {
     constexpr int e = decltype(ei)::value;
-    f[e] = St::ld(fin + Lay::template index<Q, e>(D, r));
+    f[e] = St::ld(src_ptr<Q, e, Lay>(D, r, fin));
   });
   if constexpr (BULK) return;  // a wave without boundary sites
   if (!r.cut) return;
@@ -724,9 +932,9 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
       if constexpr (sizeof(S) >= 4) {
         opp[e] = T(0);
         if (link && !((T)(double)q[dp - 1] < T(0.5)))
-          opp[e] = St::ld(fin + Lay::template at<Q>(D, e, x));
+          opp[e] = St::ld(own_ptr<Q, Lay>(D, r, fin, e));
       } else {
-        opp[e] = St::ld(fin + Lay::template at<Q>(D, link ? e : dp, x));
+        opp[e] = St::ld(own_ptr<Q, Lay>(D, r, fin, link ? e : dp));
       }
     });
     sfor<1, Q>([&](auto ei) {
@@ -815,11 +1023,14 @@ __device__ __forceinline__ void step_site(const DevDomain& D, const StepArgs& A,
   // (StabilityTester.h:97-113), not inside the step.
   StepArgs a = A;
   Lay::relax(D, r, a);
-  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, a);
+  const Hydro<Q, T> h = collide<Q, KERNEL, S>(f, a);
   update_stash<Q, T, BULK>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
-    store_out<S>(fout + Lay::template at<Q>(D, d, r.x), f[d]);
+    if constexpr (std::is_same<Lay, BlockLayout32>::value)
+      Lay::template store<Q, S>(D, r, fout, d, (S)f[d]);
+    else
+      store_out<S>(own_ptr<Q, Lay>(D, r, fout, d), f[d]);
   });
 }
 
@@ -833,6 +1044,10 @@ __global__ __launch_bounds__(256) void k_step(DevDomain D, StepArgs A,
   if constexpr (kWaveLayout<Lay>) {
     const uint32_t g = step_workgroup(A.xcd_remap);
     const uint32_t w = __builtin_amdgcn_readfirstlane(g * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    // (A wave-uniform branch to the boundary-free code for planes without
+    // boundary sites cuts their instructions by a third but measured slower:
+    // the kernel then needs more registers and the two paths more
+    // instruction cache.)
     if (!Lay::locate_wave(D, w, r)) return;
   } else {
     if (!Lay::locate(D, r)) return;
@@ -897,7 +1112,7 @@ __global__ __launch_bounds__(256) void k_collide_only(DevDomain D, StepArgs A,
   });
   StepArgs a = A;
   Lay::relax(D, r, a);
-  const Hydro<Q, T> h = collide<Q, KERNEL, kShifted<S>, T>(f, a);
+  const Hydro<Q, T> h = collide<Q, KERNEL, S>(f, a);
   update_stash<Q, T>(D, r, h);
   sfor<0, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
@@ -1522,61 +1737,83 @@ int launch_fill(const DevDomain& d, const StepArgs& a, const double* vals, void*
   });
 }
 
+namespace {
+
+// One step on the launch shape in `a` with layout Lay.
+template <int Q, int K, typename S, class Lay>
+int step_launch(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
+                void* stream) {
+  if constexpr (kWaveLayout<Lay>) {
+    // launch shape: a.threads per workgroup, a.planes per wave, a grid of
+    // a.groups workgroups (0: enough to cover every plane once)
+    const uint32_t wpg = a.threads / 64u;
+    const uint64_t need = ((uint64_t)d.nwaves + (uint64_t)wpg * a.planes - 1) /
+                          ((uint64_t)wpg * a.planes);
+    uint64_t groups = a.groups ? std::min<uint64_t>(a.groups, need) : need;
+    const bool one_pass = a.planes == 1 && groups == need;
+    if (one_pass && a.split && d.bulk_waves && d.bnd_waves) {
+      // bulk planes first, then the planes with boundary sites
+      const uint32_t counts[2] = {d.n_bulk_waves, d.n_bnd_waves};
+      const uint32_t* lists[2] = {d.bulk_waves, d.bnd_waves};
+      for (int part = 0; part < 2; ++part) {
+        if (!counts[part]) continue;
+        uint64_t gp = ((uint64_t)counts[part] + wpg - 1) / wpg;
+        if (a.xcd_remap >= 2) {
+          const uint64_t m = 8ull * a.xcd_remap;
+          gp = (gp + m - 1) / m * m;
+        }
+        const dim3 gridp((unsigned)gp), blockp(a.threads);
+        if (part == 0)
+          hipLaunchKernelGGL((k_step_list<Q, K, S, Lay, true>),
+                             gridp, blockp, 0, (hipStream_t)stream, d, a, (const S*)fin,
+                             (S*)fout, lists[part], counts[part]);
+        else
+          hipLaunchKernelGGL((k_step_list<Q, K, S, Lay, false>),
+                             gridp, blockp, 0, (hipStream_t)stream, d, a, (const S*)fin,
+                             (S*)fout, lists[part], counts[part]);
+      }
+      return (int)hipGetLastError();
+    }
+    if (a.xcd_remap >= 2) {
+      // runs of a.xcd_remap workgroups per XCD: pad the grid to whole
+      // groups of 8 runs; padded workgroups see planes >= nwaves and exit
+      const uint64_t m = 8ull * a.xcd_remap;
+      groups = (groups + m - 1) / m * m;
+    }
+    const dim3 grid((unsigned)std::max<uint64_t>(groups, 1)), block(a.threads);
+    if (one_pass)
+      hipLaunchKernelGGL((k_step<Q, K, S, Lay>), grid,
+                         block, 0, (hipStream_t)stream, d, a, (const S*)fin, (S*)fout);
+    else
+      hipLaunchKernelGGL((k_step_loop<Q, K, S, Lay>), grid,
+                         block, 0, (hipStream_t)stream, d, a, (const S*)fin, (S*)fout);
+  } else {
+    hipLaunchKernelGGL((k_step<Q, K, S, Lay>),
+                       grid_for(thread_count(d)), dim3(256), 0, (hipStream_t)stream, d, a,
+                       (const S*)fin, (S*)fout);
+  }
+  return (int)hipGetLastError();
+}
+
+// Whether every state element of the block layout is reachable by a 32-bit
+// byte offset (and the launch shape does not ask for 64-bit addressing).
+inline bool narrow_offsets(const DevDomain& d, const StepArgs& a, size_t esize) {
+  return !a.wide && (uint64_t)a.Q * (uint64_t)d.np * esize <= 0xFFFFFFFFull;
+}
+
+}  // namespace
+
 int launch_step(const DevDomain& d, const StepArgs& a, const void* fin, void* fout,
                 void* stream) {
   return dispatch(d, a, [&](auto q, auto k, auto* s, auto* lay) {
     using S = std::remove_pointer_t<decltype(s)>;
     using Lay = std::remove_pointer_t<decltype(lay)>;
-    if constexpr (kWaveLayout<Lay>) {
-      // launch shape: a.threads per workgroup, a.planes per wave, a grid of
-      // a.groups workgroups (0: enough to cover every plane once)
-      const uint32_t wpg = a.threads / 64u;
-      const uint64_t need = ((uint64_t)d.nwaves + (uint64_t)wpg * a.planes - 1) /
-                            ((uint64_t)wpg * a.planes);
-      uint64_t groups = a.groups ? std::min<uint64_t>(a.groups, need) : need;
-      const bool one_pass = a.planes == 1 && groups == need;
-      if (one_pass && a.split && d.bulk_waves && d.bnd_waves) {
-        // bulk planes first, then the planes with boundary sites
-        const uint32_t counts[2] = {d.n_bulk_waves, d.n_bnd_waves};
-        const uint32_t* lists[2] = {d.bulk_waves, d.bnd_waves};
-        for (int part = 0; part < 2; ++part) {
-          if (!counts[part]) continue;
-          uint64_t gp = ((uint64_t)counts[part] + wpg - 1) / wpg;
-          if (a.xcd_remap >= 2) {
-            const uint64_t m = 8ull * a.xcd_remap;
-            gp = (gp + m - 1) / m * m;
-          }
-          const dim3 gridp((unsigned)gp), blockp(a.threads);
-          if (part == 0)
-            hipLaunchKernelGGL((k_step_list<decltype(q)::value, decltype(k)::value, S, Lay, true>),
-                               gridp, blockp, 0, (hipStream_t)stream, d, a, (const S*)fin,
-                               (S*)fout, lists[part], counts[part]);
-          else
-            hipLaunchKernelGGL((k_step_list<decltype(q)::value, decltype(k)::value, S, Lay, false>),
-                               gridp, blockp, 0, (hipStream_t)stream, d, a, (const S*)fin,
-                               (S*)fout, lists[part], counts[part]);
-        }
-        return (int)hipGetLastError();
-      }
-      if (a.xcd_remap >= 2) {
-        // runs of a.xcd_remap workgroups per XCD: pad the grid to whole
-        // groups of 8 runs; padded workgroups see planes >= nwaves and exit
-        const uint64_t m = 8ull * a.xcd_remap;
-        groups = (groups + m - 1) / m * m;
-      }
-      const dim3 grid((unsigned)std::max<uint64_t>(groups, 1)), block(a.threads);
-      if (one_pass)
-        hipLaunchKernelGGL((k_step<decltype(q)::value, decltype(k)::value, S, Lay>), grid,
-                           block, 0, (hipStream_t)stream, d, a, (const S*)fin, (S*)fout);
-      else
-        hipLaunchKernelGGL((k_step_loop<decltype(q)::value, decltype(k)::value, S, Lay>), grid,
-                           block, 0, (hipStream_t)stream, d, a, (const S*)fin, (S*)fout);
-    } else {
-      hipLaunchKernelGGL((k_step<decltype(q)::value, decltype(k)::value, S, Lay>),
-                         grid_for(thread_count(d)), dim3(256), 0, (hipStream_t)stream, d, a,
-                         (const S*)fin, (S*)fout);
+    constexpr int Q = decltype(q)::value, K = decltype(k)::value;
+    if constexpr (std::is_same<Lay, BlockLayout>::value) {
+      if (narrow_offsets(d, a, sizeof(S)))
+        return step_launch<Q, K, S, BlockLayout32>(d, a, fin, fout, stream);
     }
-    return (int)hipGetLastError();
+    return step_launch<Q, K, S, Lay>(d, a, fin, fout, stream);
   });
 }
 

@@ -1568,6 +1568,7 @@ static int check_shape(const hlb_solver* s, const hlb_launch_shape& sh) {
   if (sh.xcd_remap < 0 || sh.xcd_remap > 4096)
     return fail(HLB_E_INVALID, "xcd_remap must be in 0..4096");
   if (sh.split != 0 && sh.split != 1) return fail(HLB_E_INVALID, "split must be 0 or 1");
+  if (sh.wide != 0 && sh.wide != 1) return fail(HLB_E_INVALID, "wide must be 0 or 1");
   return HLB_OK;
 }
 
@@ -1580,6 +1581,7 @@ int hlb_set_launch_shape(hlb_solver* s, const hlb_launch_shape* shape) {
   s->args.groups = (uint32_t)shape->groups;
   s->args.xcd_remap = (uint32_t)shape->xcd_remap;
   s->args.split = (uint32_t)shape->split;
+  s->args.wide = (uint32_t)shape->wide;
   return HLB_OK;
 }
 
@@ -1590,6 +1592,7 @@ int hlb_get_launch_shape(const hlb_solver* s, hlb_launch_shape* shape) {
   shape->groups = (int32_t)s->args.groups;
   shape->xcd_remap = (int32_t)s->args.xcd_remap;
   shape->split = (int32_t)s->args.split;
+  shape->wide = (int32_t)s->args.wide;
   return HLB_OK;
 }
 
@@ -1633,6 +1636,7 @@ int hlb_autotune(hlb_solver* s, int32_t steps, hlb_launch_shape* best, double* b
   cand.push_back({256, 2, cus * 24 * 64 / 256, 1, 0});
   hlb_launch_shape keep{};
   hlb_get_launch_shape(s, &keep);
+  for (auto& c : cand) c.wide = keep.wide;  // the addressing stays the caller's
   double best_t = 1e300;
   hlb_launch_shape best_s = keep;
   int rc = HLB_OK;

@@ -429,11 +429,13 @@ class LBM:
             self._h, ref.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
         return out
 
-    def set_launch_shape(self, threads=256, planes=1, groups=0, xcd_remap=0, split=0) -> None:
+    def set_launch_shape(self, threads=256, planes=1, groups=0, xcd_remap=0, split=0,
+                         wide=0) -> None:
         """xcd_remap: 0 hardware workgroup order, 1 one contiguous range per
         XCD, K in 2..4096 runs of K workgroups per XCD; split: planes with
-        and without boundary sites in two launches (one-pass shapes)."""
-        sh = N.hlb_launch_shape(threads, planes, groups, int(xcd_remap), int(split))
+        and without boundary sites in two launches (one-pass shapes); wide:
+        64-bit element addressing even for states below 4 GiB."""
+        sh = N.hlb_launch_shape(threads, planes, groups, int(xcd_remap), int(split), int(wide))
         N.check(N.lib().hlb_set_launch_shape(self._h, C.byref(sh)))
 
     @property
@@ -441,7 +443,7 @@ class LBM:
         sh = N.hlb_launch_shape()
         N.check(N.lib().hlb_get_launch_shape(self._h, C.byref(sh)))
         return {"threads": sh.threads, "planes": sh.planes, "groups": sh.groups,
-                "xcd_remap": sh.xcd_remap, "split": sh.split}
+                "xcd_remap": sh.xcd_remap, "split": sh.split, "wide": sh.wide}
 
     def autotune(self, steps: int = 20) -> dict:
         """Times the candidate launch shapes on the current state (which is
@@ -449,7 +451,8 @@ class LBM:
         sh, ms = N.hlb_launch_shape(), C.c_double()
         N.check(N.lib().hlb_autotune(self._h, int(steps), C.byref(sh), C.byref(ms)))
         return {"threads": sh.threads, "planes": sh.planes, "groups": sh.groups,
-                "xcd_remap": sh.xcd_remap, "split": sh.split, "ms_per_step": ms.value}
+                "xcd_remap": sh.xcd_remap, "split": sh.split, "wide": sh.wide,
+                "ms_per_step": ms.value}
 
     def snapshot_velocity(self) -> None:
         N.check(N.lib().hlb_snapshot_velocity(self._h))

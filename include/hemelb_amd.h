@@ -268,6 +268,9 @@ typedef struct {
   int32_t split;       /* one-pass shapes: 1 runs the planes without boundary
                           sites and those with them as two launches of
                           separately compiled kernels, 0 one launch */
+  int32_t wide;        /* 1: 64-bit element addressing even where 32-bit
+                          byte offsets reach the whole state (below 4 GiB
+                          per state buffer, where 0 uses them) */
 } hlb_launch_shape;
 int hlb_set_launch_shape(hlb_solver* s, const hlb_launch_shape* shape);
 int hlb_get_launch_shape(const hlb_solver* s, hlb_launch_shape* shape);

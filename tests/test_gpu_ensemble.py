@@ -92,7 +92,7 @@ def test_autotune_restores_the_state(ensemble):
     assert lbm.GetTimeStep() == t
     np.testing.assert_array_equal(lbm.get_distributions(), before)
     assert lbm.launch_shape == {k: best[k] for k in ("threads", "planes", "groups", "xcd_remap",
-                                                     "split")}
+                                                     "split", "wide")}
     other = e.solver(storage="fp32")
     other.step(5)
     lbm.step(10)
