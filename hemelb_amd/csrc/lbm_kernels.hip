@@ -97,6 +97,12 @@ __host__ __device__ constexpr int inv_dir(int d) {
   return d == 0 ? 0 : ((d & 1) ? d + 1 : d - 1);
 }
 
+// Link masks hold direction d in bit d-1, so a pair (d, d+1) sits in bits
+// (d-1, d): bit d-1 of pair_swap(x) is bit inv_dir(d)-1 of x.
+__host__ __device__ constexpr uint32_t pair_swap(uint32_t x) {
+  return ((x & 0x55555555u) << 1) | ((x >> 1) & 0x55555555u);
+}
+
 constexpr double kCs2 = 1.0 / 3.0;  // constants.h:41
 
 template <int I, int N, typename F>
@@ -797,6 +803,25 @@ constexpr bool kWaveLayout = std::is_same<Lay, BlockLayout>::value ||
                              std::is_same<Lay, BlockLayout32>::value ||
                              std::is_same<Lay, CompactBlockLayout>::value;
 
+// Own slot E of the lane's site where pick_e, else its own slot DP (a
+// per-lane choice between two directions).  BlockLayout32: the wave-uniform
+// offset of the lower direction plus a per-lane choice of two non-negative
+// constants.
+template <int Q, int E, int DP, class Lay, typename P>
+__device__ __forceinline__ P* own_pick_ptr(const DevDomain& D, const SiteRef& r, P* base,
+                                           bool pick_e) {
+  if constexpr (std::is_same<Lay, BlockLayout32>::value) {
+    constexpr uint32_t ES = sizeof(P);
+    constexpr int LO = E < DP ? E : DP;
+    constexpr uint32_t dE = (uint32_t)(E - LO) * 512u * ES, dP = (uint32_t)(DP - LO) * 512u * ES;
+    const uint32_t uni = (((r.w >> 3) * (uint32_t)Q + (uint32_t)LO) * 512u + ((r.w & 7u) << 6)) * ES;
+    using C = std::conditional_t<std::is_const<P>::value, const char, char>;
+    return (P*)(((C*)base + uni) + (r.lane * ES + (pick_e ? dE : dP)));
+  } else {
+    return base + Lay::template at<Q>(D, pick_e ? E : DP, r.x);
+  }
+}
+
 // Addresses of state elements: slot d of the lane's own site, and the value
 // streaming into slot E of it (its source, or its own inverse slot across a
 // cut link).  Element indices for the 64-bit layouts, byte offsets for
@@ -923,27 +948,27 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
     // off; measured 0.3-0.6 % faster on the vessel tree).  Binary16: every
     // lane loads, the unused ones their already-fetched bounce-back slot
     // (predicated 16-bit loads measured 10 % slower).
+    // Slot e is a candidate when the inverse link is a wall link and not an
+    // iolet link, and link e itself is not a wall link: one mask for all e.
+    const uint32_t cand = pair_swap(wm) & ~pair_swap(im) & ~wm;
     T opp[Q];
     sfor<1, Q>([&](auto ei) {
       constexpr int e = decltype(ei)::value;
       constexpr int dp = inv_dir(e);
-      const bool link = ((wm >> (dp - 1)) & 1u) && !((im >> (dp - 1)) & 1u) &&
-                        !((wm >> (e - 1)) & 1u);
+      const bool link = (cand >> (e - 1)) & 1u;
       if constexpr (sizeof(S) >= 4) {
         opp[e] = T(0);
         if (link && !((T)(double)q[dp - 1] < T(0.5)))
           opp[e] = St::ld(own_ptr<Q, Lay>(D, r, fin, e));
       } else {
-        opp[e] = St::ld(own_ptr<Q, Lay>(D, r, fin, link ? e : dp));
+        opp[e] = St::ld(own_pick_ptr<Q, e, dp, Lay>(D, r, fin, link));
       }
     });
     sfor<1, Q>([&](auto ei) {
       constexpr int e = decltype(ei)::value;
       constexpr int dp = inv_dir(e);
       const T qd = (T)(double)q[dp - 1];
-      if (((wm >> (dp - 1)) & 1u) && !((im >> (dp - 1)) & 1u) && !((wm >> (e - 1)) & 1u) &&
-          !(qd < T(0.5)))
-      {
+      if (((cand >> (e - 1)) & 1u) && !(qd < T(0.5))) {
         if constexpr (kShifted<S>)
           f[e] = mad(T(2.0) * qd - T(1), opp[e], f[e]) * recip(T(2.0) * qd);
         else
@@ -970,13 +995,17 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
     });
   }
   if (A.wall == 1 && wm) {
-    // BouzidiFirdaousLallemandLink::PostStepLink
+    // BouzidiFirdaousLallemandLink::PostStepLink: links d with a wall and
+    // no wall on the inverse link, q < 0.5
+    const uint32_t post = wm & ~pair_swap(wm);
     sfor<1, Q>([&](auto di) {
       constexpr int d = decltype(di)::value;
       constexpr int id = inv_dir(d);
-      if (wm & (1u << (d - 1))) {
-        const T qd = (T)(double)q[d - 1];
-        if (!(wm & (1u << (id - 1))) && qd < T(0.5))
+      const T qd = (T)(double)q[d - 1];
+      if (((post >> (d - 1)) & 1u) && qd < T(0.5)) {
+        if constexpr (kShifted<S>)
+          f[id] = mad(T(2.0) * qd, f[id] - f[d], f[d]);
+        else
           f[id] = T(2.0) * qd * f[id] + (T(1.0) - T(2.0) * qd) * f[d];
       }
     });
