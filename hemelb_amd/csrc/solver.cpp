@@ -196,6 +196,35 @@ uint64_t spread3(uint64_t v) {
   return x;
 }
 
+// Slot of a site inside its 8^3 block: x plane major, then the in-plane
+// lane, the order the block-layout kernels address (lbm_kernels.hip
+// kMortonLanes): with `morton` (binary64 storage) the bits of (y, z)
+// interleaved, z in bits 0, 2, 4 and y in bits 1, 3, 5; otherwise rows of z
+// (y * 8 + z).
+inline uint32_t plane_lane(uint64_t y, uint64_t z, bool morton) {
+  if (!morton) return (uint32_t)((y << 3) | z);
+  uint32_t j = 0;
+  for (int b = 0; b < 3; ++b)
+    j |= (uint32_t)(((z >> b) & 1u) << (2 * b)) | (uint32_t)(((y >> b) & 1u) << (2 * b + 1));
+  return j;
+}
+inline uint32_t block_slot(uint64_t x, uint64_t y, uint64_t z, bool morton) {
+  return (uint32_t)((x & 7) << 6) | plane_lane(y & 7, z & 7, morton);
+}
+// (y, z) inside the block of in-plane lane j
+inline void lane_yz(uint32_t j, bool morton, int& y, int& z) {
+  if (!morton) {
+    y = (int)(j >> 3);
+    z = (int)(j & 7);
+    return;
+  }
+  y = z = 0;
+  for (int b = 0; b < 3; ++b) {
+    z |= (int)((j >> (2 * b)) & 1u) << b;
+    y |= (int)((j >> (2 * b + 1)) & 1u) << b;
+  }
+}
+
 // Absolute coordinate key of a site (21 bits per axis, offset 2^20).
 inline uint64_t coord_key(const int32_t* c) {
   return ((uint64_t)((int64_t)c[0] + (1 << 20)) << 42) |
@@ -549,7 +578,9 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
         }
   }
 
-  // Internal order: 8^3 blocks in Morton order, z-fastest inside a block.
+  // Internal order: 8^3 blocks in Morton order; inside a block x planes,
+  // then the storage's in-plane lane order (block_slot).
+  const bool morton = params->storage == HLB_STORE_FP64;
   int32_t lo[3];
   for (int a = 0; a < 3; ++a) lo[a] = dm->coords[a];
   for (int64_t i = 0; i < n; ++i)
@@ -561,7 +592,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
                    y = (uint64_t)(dm->coords[3 * i + 1] - lo[1]),
                    z = (uint64_t)(dm->coords[3 * i + 2] - lo[2]);
     const uint64_t oct = (spread3(x >> 3) << 2) | (spread3(y >> 3) << 1) | spread3(z >> 3);
-    return (oct << 9) | (((x & 7) << 6) | ((y & 7) << 3) | (z & 7));
+    return (oct << 9) | block_slot(x, y, z, morton);
   };
   std::vector<std::pair<uint64_t, int64_t>> keys(n);
   for (int64_t i = 0; i < n; ++i) keys[i] = {key_of(i), i};
@@ -589,7 +620,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
                    y = (uint64_t)(gh->coords[3 * g + 1] - lo[1]),
                    z = (uint64_t)(gh->coords[3 * g + 2] - lo[2]);
     const uint64_t oct = (spread3(x >> 3) << 2) | (spread3(y >> 3) << 1) | spread3(z >> 3);
-    gkeys[g] = {(oct << 9) | (((x & 7) << 6) | ((y & 7) << 3) | (z & 7)), g};
+    gkeys[g] = {(oct << 9) | block_slot(x, y, z, morton), g};
   }
   std::sort(gkeys.begin(), gkeys.end());
   for (int64_t g = 1; g < ng; ++g)
@@ -615,14 +646,14 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
                   z = dm->coords[3 * i + 2] - lo[2];
     auto& b = blocks[bkey(x >> 3, y >> 3, z >> 3)];
     if (b.empty()) b.assign(512, -1);
-    b[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)] = (int32_t)o;
+    b[block_slot(x, y, z, morton)] = (int32_t)o;
   }
   for (int64_t g = 0; g < ng; ++g) {
     const int64_t x = s->ghost_coords[3 * g] - lo[0], y = s->ghost_coords[3 * g + 1] - lo[1],
                   z = s->ghost_coords[3 * g + 2] - lo[2];
     auto& b = blocks[bkey(x >> 3, y >> 3, z >> 3)];
     if (b.empty()) b.assign(512, -1);
-    int32_t& slot = b[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)];
+    int32_t& slot = b[block_slot(x, y, z, morton)];
     if (slot >= 0) return fail(HLB_E_INVALID, "a ghost site is also a site of the domain");
     slot = (int32_t)(n + g);
   }
@@ -630,7 +661,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
     if (x < 0 || y < 0 || z < 0) return -1;
     auto it = blocks.find(bkey(x >> 3, y >> 3, z >> 3));
     if (it == blocks.end()) return -1;
-    return it->second[((x & 7) << 6) | ((y & 7) << 3) | (z & 7)];
+    return it->second[block_slot(x, y, z, morton)];
   };
 
   // Blocks of the compact order (it is sorted by Morton block key).
@@ -818,7 +849,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
     h_wc.assign(nwaves, 0);
     uint32_t rec = 0;
     int64_t o = 0;
-    // block slot of internal site o (x plane major, z fastest)
+    // block slot of internal site o (x plane major, then block_slot's lanes)
     auto bslot = [&](int64_t q) { return (blk_of[q] << 9) | (int64_t)(keys[q].first & 511); };
     for (int64_t w = 0; w < nwaves; ++w) {
       h_wr[w] = rec;
@@ -846,7 +877,9 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
       const uint32_t cutm = dm->wall_mask[i] | dm->iolet_mask[i];
       const int64_t slot = bslot(o2);
       const int64_t w = slot >> 6, lane = slot & 63;
-      const int lx = (int)(w & 7), ly = (int)(lane >> 3), lz = (int)(lane & 7);
+      const int lx = (int)(w & 7);
+      int ly, lz;
+      lane_yz((uint32_t)lane, morton, ly, lz);
       const int64_t x = dm->coords[3 * i] - lo[0], y = dm->coords[3 * i + 1] - lo[1],
                     z = dm->coords[3 * i + 2] - lo[2];
       for (int e = 1; e < Q; ++e) {
@@ -860,7 +893,8 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
         const int base = 13 + 9 * ox;
         const int32_t nb = cy ? (cz ? nbt[base - 3 * CY - CZ] : nbt[base - 3 * CY])
                               : (cz ? nbt[base - CZ] : nbt[base]);
-        const int sl = (((ly - CY) & 7) << 3) | ((lz - CZ) & 7);
+        const int sl =
+            (int)plane_lane((uint64_t)((ly - CY) & 7), (uint64_t)((lz - CZ) & 7), morton);
         int64_t src = ((int64_t)nb << 9) + (((sx & 7) << 6) | sl);
         if (compact_store && nb >= 0) {
           // CompactBlockLayout::offset: plane base + fluid lanes below
