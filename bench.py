@@ -26,6 +26,11 @@ chosen by the solver's autotuner, over the workload's 4000 steps.
 (LBGK and MRT) to their full step counts in every storage form and prints one
 JSON line per (workload, form) with the velocity, pressure and wall shear
 stress errors against the binary64 run.
+
+``--shape-sweep`` instead times the step of every storage form on the vessel
+tree at each launch shape (64/128/256 threads per workgroup, workgroup-to-XCD
+remaps 0, 1, 2, 8, 32) and prints one JSON line per (form, shape): the best
+of three timings of --steps steps after --warmup steps.
 """
 from __future__ import annotations
 
@@ -147,6 +152,26 @@ def precision_report(args) -> int:
     return 0
 
 
+def shape_sweep(args) -> int:
+    from hemelb_amd import workloads as WL
+
+    w = WL.vessel_tree("LBGK")
+    n = w.domain.n
+    shapes = [(t, r) for t in (128, 256) for r in (0, 1, 2, 8, 32)] + [(64, 0), (64, 8)]
+    for storage in ("fp64", "fp32", "fp16"):
+        lbm = w.solver(storage=storage)
+        lbm.step(5)
+        for threads, remap in shapes:
+            lbm.set_launch_shape(threads=threads, planes=1, groups=0, xcd_remap=remap, split=0)
+            lbm.time_steps(args.warmup)
+            ms = min(lbm.time_steps(args.steps) / args.steps for _ in range(3))
+            print(json.dumps({"workload": w.name, "storage": storage, "threads": threads,
+                              "xcd_remap": remap, "ms_per_step": round(ms, 5),
+                              "mlups": round(n / (ms * 1e-3) / 1e6, 1)}), flush=True)
+        del lbm
+    return 0
+
+
 def ensemble_bench(args) -> int:
     from hemelb_amd import workloads as WL
 
@@ -222,6 +247,8 @@ def main() -> int:
     ap.add_argument("--form-steps", type=int, default=0,
                     help="timed steps of each other storage form (0: as many as --steps)")
     ap.add_argument("--precision-report", action="store_true")
+    ap.add_argument("--shape-sweep", action="store_true",
+                    help="time every storage form at each launch shape")
     ap.add_argument("--report-steps", type=int, default=0,
                     help="steps per precision-report run (default: the workload's)")
     ap.add_argument("--report-workloads", default="")
@@ -240,6 +267,8 @@ def main() -> int:
 
     if args.precision_report:
         return precision_report(args)
+    if args.shape_sweep:
+        return shape_sweep(args)
     if args.ensemble:
         return ensemble_bench(args)
 
