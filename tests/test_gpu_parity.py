@@ -305,3 +305,29 @@ def test_split_launch_reduced_storage_is_bitexact(storage, layout):
         one.step(n)
         two.step(n)
         np.testing.assert_array_equal(two.get_distributions(), one.get_distributions())
+
+
+def test_wide_addressing_is_bitexact():
+    """The block layout's step with 64-bit element indices (the path of
+    states of 4 GiB and more) against the oracle, in every storage form
+    against the default 32-bit byte offsets: the test domains are small, so
+    the launch shape's `wide` flag selects that path here."""
+    case = K.tree_case(3, 6.0, seed=1)
+    print(f"[parity] wide addressing n={case.domain.n}", flush=True)
+    orc = case.oracle()
+    wide = case.product(storage="fp64")
+    wide.set_launch_shape(threads=128, xcd_remap=2, wide=1)
+    assert wide.launch_shape["wide"] == 1
+    for n in (1, 2, 17):
+        orc.step(n)
+        wide.step(n)
+        np.testing.assert_array_equal(wide.get_distributions(), orc.get_f())
+    for storage in ("fp32", "fp16"):
+        narrow = case.product(storage=storage)
+        wide = case.product(storage=storage)
+        wide.set_launch_shape(threads=128, xcd_remap=8, wide=1)
+        for n in (1, 2, 17):
+            narrow.step(n)
+            wide.step(n)
+            np.testing.assert_array_equal(wide.get_distributions(),
+                                          narrow.get_distributions(), storage)
