@@ -47,9 +47,10 @@ struct DevDomain {
   const uint32_t* b_wall = nullptr;   // [nb] wall link mask
   const uint32_t* b_iolet = nullptr;  // [nb] iolet link mask
   const int32_t* b_io = nullptr;      // [nb] iolet-site record or -1
-  const float* b_q = nullptr;         // [nb][QS] cut distances of links 1..Q-1,
-                                      // QS = Q-1 rounded up to a multiple of 4
-                                      // (binary64 storage)
+  const float* b_q = nullptr;         // binary64 storage: the cut distances
+                                      // of each record's wall links, packed
+                                      // in direction order
+  const uint32_t* b_qoff = nullptr;   // [nb] first entry of record r in b_q
   const uint16_t* b_q16 = nullptr;    // [nb][QS16] the same as q * 65535,
                                       // QS16 = Q-1 rounded up to a multiple of 8
                                       // (reduced-precision storage)

@@ -269,6 +269,7 @@ struct hlb_solver {
   DevBuf<uint32_t> binfo, b_wall, b_iolet;
   DevBuf<int32_t> b_io, io_iolet, io_bc, io_site, iolet_kind;
   DevBuf<float> b_q, iolet_nf;
+  DevBuf<uint32_t> b_qoff;
   DevBuf<uint16_t> b_q16;
   DevBuf<double> io_stash, io_ladd, iolet_n, weights;
   DevBuf<int32_t> bnbr;
@@ -340,7 +341,7 @@ struct hlb_solver {
     int64_t b = 2 * (int64_t)storage_bytes(p.storage) * Q * np;
     b += src.bytes() + binfo.bytes() + b_wall.bytes() + b_iolet.bytes() +
          b_io.bytes() + io_iolet.bytes() + io_bc.bytes() + io_site.bytes() +
-         iolet_kind.bytes() + b_q.bytes() + b_q16.bytes() + iolet_nf.bytes() + io_stash.bytes() +
+         iolet_kind.bytes() + b_q.bytes() + b_qoff.bytes() + b_q16.bytes() + iolet_nf.bytes() + io_stash.bytes() +
          io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
          bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() +
          wave_rec.bytes() +
@@ -716,7 +717,8 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
   }
   std::vector<uint32_t> h_bwall, h_biolet;
   std::vector<int32_t> h_bio, h_io_iolet, h_io_bc, h_io_site;
-  std::vector<float> h_bq;
+  std::vector<float> h_bq;        // binary64: the wall links' cut distances, packed
+  std::vector<uint32_t> h_bqoff;  // binary64: first entry of each record in h_bq
   std::vector<uint16_t> h_bq16;  // reduced-precision storage: q * 65535
   std::vector<double> h_ladd;
   for (int64_t o = 0; o < n; ++o) {
@@ -757,20 +759,25 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
       if (!blocks_layout) h_binfo[o] = rec + 1;
       h_bwall.push_back(wm);
       h_biolet.push_back(im);
-      // Cut distances of links 1..Q-1 in 16-byte aligned records: binary32
-      // for binary64 storage (the reference's float q, bit for bit);
-      // q * 65535 rounded to 16 bits for the reduced-precision forms (a wall
-      // position error below 8e-6 lattice units, half the bytes).
+      // Cut distances (the BFL links read those of wall links only):
+      // * binary64 storage: the reference's float q, bit for bit, of the
+      //   wall links alone, packed in direction order from h_bqoff[rec];
+      // * reduced-precision forms: q * 65535 rounded to 16 bits for every
+      //   link, in 16-byte aligned records (a wall position error below 8e-6
+      //   lattice units).
       const bool reduced = P.storage != HLB_STORE_FP64;
-      const int qs = reduced ? (Q - 1 + 7) / 8 * 8 : (Q - 1 + 3) / 4 * 4;
-      for (int d = 1; d <= qs; ++d) {
-        const float q = d < Q && dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1]
-                                                  : -1.0f;
-        if (reduced)
+      if (reduced) {
+        for (int d = 1; d <= (Q - 1 + 7) / 8 * 8; ++d) {
+          const float q = d < Q && dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1]
+                                                    : -1.0f;
           h_bq16.push_back((uint16_t)std::lround((double)std::min(1.0f, std::max(0.0f, q)) *
                                                  65535.0));
-        else
-          h_bq.push_back(q);
+        }
+      } else {
+        h_bqoff.push_back((uint32_t)h_bq.size());
+        for (int d = 1; d < Q; ++d)
+          if ((wm >> (d - 1)) & 1u)
+            h_bq.push_back(dm->cut_distance ? dm->cut_distance[i * (Q - 1) + d - 1] : -1.0f);
       }
       if (im) {
         const int g = dm->site_type[i] == HLB_SITE_OUTLET ? 1 : 0;
@@ -977,6 +984,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
   HIP_OR_FAIL(s->b_iolet.upload(h_biolet));
   HIP_OR_FAIL(s->b_io.upload(h_bio));
   HIP_OR_FAIL(s->b_q.upload(h_bq));
+  HIP_OR_FAIL(s->b_qoff.upload(h_bqoff));
   HIP_OR_FAIL(s->b_q16.upload(h_bq16));
   HIP_OR_FAIL(s->io_iolet.upload(h_io_iolet));
   HIP_OR_FAIL(s->io_bc.upload(h_io_bc));
@@ -1036,6 +1044,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
   D.b_iolet = s->b_iolet.p;
   D.b_io = s->b_io.p;
   D.b_q = s->b_q.p;
+  D.b_qoff = s->b_qoff.p;
   D.b_q16 = s->b_q16.p;
   D.ni = (int64_t)h_io_iolet.size();
   D.io_iolet = s->io_iolet.p;
