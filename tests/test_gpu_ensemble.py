@@ -71,8 +71,10 @@ def test_launch_shapes_agree_bitwise(ensemble):
               dict(threads=128, xcd_remap=64), dict(groups=9, xcd_remap=4),
               dict(planes=2, xcd_remap=8), dict(threads=64, xcd_remap=3),
               dict(groups=11, xcd_remap=5), dict(threads=128, xcd_remap=8, split=0),
-              dict(threads=256, split=0), dict(threads=64, xcd_remap=16, split=1)]
-    for bad in (dict(xcd_remap=4097), dict(xcd_remap=-1), dict(split=2)):
+              dict(threads=256, split=0), dict(threads=64, xcd_remap=16, split=1),
+              dict(threads=128, xcd_remap=2, wide=1), dict(planes=2, wide=1),
+              dict(threads=64, split=1, wide=1)]
+    for bad in (dict(xcd_remap=4097), dict(xcd_remap=-1), dict(split=2), dict(wide=2)):
         with pytest.raises(Exception):
             ref.set_launch_shape(**bad)
     for sh in shapes:
