@@ -288,7 +288,10 @@ int hlbn_create(int32_t side, int32_t batch, int32_t in_channels, int32_t out_ch
     max_part = std::max(max_part, (P + ch - 1) / ch * c.co);
     max_pc = std::max(max_pc, P * c.co);
   }
-  n->head_chunk = std::max<int64_t>(256, (n->P[0] + 1023) / 1024);
+  // positions per partial sum of the head's weight gradient and of the
+  // per-channel sums at full resolution: 16384 chunks of 64 at a 64^3 x 4
+  // batch, one thread each (1024 chunks of 1024 took 175 us)
+  n->head_chunk = std::max<int64_t>(64, (n->P[0] + 16383) / 16384);
   max_part = std::max(max_part, (n->P[0] + n->head_chunk - 1) / n->head_chunk *
                                     (int64_t)out_channels * C1);
   HIPN(n->part.alloc(max_part));

@@ -649,9 +649,19 @@ __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict
 inline dim3 blocks_for(int64_t n, int t = 256) { return dim3((unsigned)((n + t - 1) / t)); }
 
 // out[i] = sum over k of part[k * n + i] (fixed order): 16 outputs per
-// block in 16 chunk phases.
+// block in 16 chunk phases, or, for few outputs (the biases, the head),
+// 4 outputs in 64 phases or one output in 256 phases, so that the grid
+// still has thousands of threads.
 void sum_chunks(const float* part, int64_t nchunks, int64_t n, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_chunks<16>, blocks_for(n, 16), dim3(256), 0, s, part, nchunks, n, out);
+  if (n >= 2048)
+    hipLaunchKernelGGL(k_sum_chunks<16>, blocks_for(n, 16), dim3(256), 0, s, part, nchunks, n,
+                       out);
+  else if (n >= 256)
+    hipLaunchKernelGGL(k_sum_chunks<4>, blocks_for(n, 4), dim3(256), 0, s, part, nchunks, n,
+                       out);
+  else
+    hipLaunchKernelGGL(k_sum_chunks<1>, blocks_for(n, 1), dim3(256), 0, s, part, nchunks, n,
+                       out);
 }
 
 }  // namespace
