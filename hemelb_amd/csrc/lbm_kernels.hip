@@ -400,82 +400,86 @@ __device__ __forceinline__ Hydro<Q, T> collide_lbgk_shifted(T (&f)[Q], const Ste
 }
 
 // Collide in place: f (post-stream, possibly shifted) -> f* ; returns the
-// moments the streamers of the next step need.
+// moments the streamers of the next step need.  Binary64 LBGK and MRT in
+// the reference's evaluation order, MRT of the shifted forms below; LBGK of
+// the shifted forms in collide_lbgk_shifted.
 template <int Q, int KERNEL, typename S, typename T = typename Store<S>::T>
 __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
   constexpr bool SHIFT = kShifted<S>;
-  if constexpr (SHIFT && KERNEL == 0)
+  if constexpr (SHIFT && KERNEL == 0) {
     return collide_lbgk_shifted<Q, T, std::is_same<S, float>::value>(f, a);
-  using L = Lat<Q>;
-  Hydro<Q, T> h;
-  T r;
-  moments<Q, T>(f, r, h.mx, h.my, h.mz);
-  T fneq[Q];
-  if constexpr (SHIFT) {
-    const T rho = T(a.rho0) + r;
-    h.rho = rho;
-    FeqShifted<T> fe(r, rho, h.mx, h.my, h.mz);
-    sfor<0, Q>([&](auto di) {
-      constexpr int d = decltype(di)::value;
-      fneq[d] = f[d] - fe.template eq<Q, d>();
-    });
   } else {
-    h.rho = r;
-    FeqCoef<T> fe(r, h.mx, h.my, h.mz);
-    sfor<0, Q>([&](auto di) {
-      constexpr int d = decltype(di)::value;
-      fneq[d] = f[d] - fe.template eq<Q, d>();
-    });
-  }
-  if constexpr (KERNEL == 0) {
-    // LBGK::Collide: f + f_neq * omega (LBGK.h:167-174)
-    const T omega = std::is_same<T, double>::value ? T(a.omega) : T(a.omega_f);
-    sfor<0, Q>([&](auto di) {
-      constexpr int d = decltype(di)::value;
-      f[d] = mad(fneq[d], omega, f[d]);
-    });
-  } else {
-    // MRT: m_neq = M f_neq (MRT.h:123-135); f* = f - sum_k (S_k N_kd) m_neq_k
-    // with N = M / diag(M M^T) (MRT.h:139-152).  S_k N_kd is a compile-time
-    // binary64 product except for the 1/tau moments, where it is formed from
-    // the runtime 1/tau exactly as the reference forms it.
-    constexpr int NM = L::nmom;
-    T mneq[NM];
-    sfor<0, NM>([&](auto ki) {
-      constexpr int k = decltype(ki)::value;
-      T acc = T(0.);
-      sfor<0, Q>([&](auto ii) {
-        constexpr int i = decltype(ii)::value;
-        constexpr int m = L::M[k][i];
-        if constexpr (m == 1) acc = acc + fneq[i];
-        else if constexpr (m == -1) acc = acc - fneq[i];
-        else if constexpr (m != 0) acc = acc + T(m) * fneq[i];
+    using L = Lat<Q>;
+    Hydro<Q, T> h;
+    T r;
+    moments<Q, T>(f, r, h.mx, h.my, h.mz);
+    T fneq[Q];
+    if constexpr (SHIFT) {
+      const T rho = T(a.rho0) + r;
+      h.rho = rho;
+      FeqShifted<T> fe(r, rho, h.mx, h.my, h.mz);
+      sfor<0, Q>([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        fneq[d] = f[d] - fe.template eq<Q, d>();
       });
-      mneq[k] = acc;
-    });
-    const double s_tau = a.mrt_s_tau;
-    sfor<0, Q>([&](auto di) {
-      constexpr int d = decltype(di)::value;
-      T coll = T(0.);
+    } else {
+      h.rho = r;
+      FeqCoef<T> fe(r, h.mx, h.my, h.mz);
+      sfor<0, Q>([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        fneq[d] = f[d] - fe.template eq<Q, d>();
+      });
+    }
+    if constexpr (KERNEL == 0) {
+      // LBGK::Collide: f + f_neq * omega (LBGK.h:167-174)
+      const T omega = std::is_same<T, double>::value ? T(a.omega) : T(a.omega_f);
+      sfor<0, Q>([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        f[d] = mad(fneq[d], omega, f[d]);
+      });
+    } else {
+      // MRT: m_neq = M f_neq (MRT.h:123-135); f* = f - sum_k (S_k N_kd) m_neq_k
+      // with N = M / diag(M M^T) (MRT.h:139-152).  S_k N_kd is a compile-time
+      // binary64 product except for the 1/tau moments, where it is formed from
+      // the runtime 1/tau exactly as the reference forms it.
+      constexpr int NM = L::nmom;
+      T mneq[NM];
       sfor<0, NM>([&](auto ki) {
         constexpr int k = decltype(ki)::value;
-        if constexpr (L::M[k][d] != 0) {
-          constexpr double N = (double)L::M[k][d] / mrt_diag<Q>(k);
-          constexpr double rate = mrt_rate(Q, k);
-          double c;
-          if constexpr (rate != 0.0) {
-            constexpr double cc = rate * N;
-            c = cc;
-          } else {
-            c = s_tau * N;
-          }
-          coll = mad(T(c), mneq[k], coll);
-        }
+        T acc = T(0.);
+        sfor<0, Q>([&](auto ii) {
+          constexpr int i = decltype(ii)::value;
+          constexpr int m = L::M[k][i];
+          if constexpr (m == 1) acc = acc + fneq[i];
+          else if constexpr (m == -1) acc = acc - fneq[i];
+          else if constexpr (m != 0) acc = acc + T(m) * fneq[i];
+        });
+        mneq[k] = acc;
       });
-      f[d] = f[d] - coll;
-    });
+      const double s_tau = a.mrt_s_tau;
+      sfor<0, Q>([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        T coll = T(0.);
+        sfor<0, NM>([&](auto ki) {
+          constexpr int k = decltype(ki)::value;
+          if constexpr (L::M[k][d] != 0) {
+            constexpr double N = (double)L::M[k][d] / mrt_diag<Q>(k);
+            constexpr double rate = mrt_rate(Q, k);
+            double c;
+            if constexpr (rate != 0.0) {
+              constexpr double cc = rate * N;
+              c = cc;
+            } else {
+              c = s_tau * N;
+            }
+            coll = mad(T(c), mneq[k], coll);
+          }
+        });
+        f[d] = f[d] - coll;
+      });
+    }
+    return h;
   }
-  return h;
 }
 
 // ---------------------------------------------------------------------------
