@@ -812,6 +812,34 @@ struct BlockLayout32 : BlockLayout {
     using C = std::conditional_t<std::is_const<P>::value, const char, char>;
     return (P*)((C*)base + off);
   }
+  // The value at x - c_E whether or not link E is cut (the gather then
+  // replaces a cut link's value): a neighbour block that does not exist is
+  // replaced by the own block, so that the address stays inside the state.
+  template <int Q, int E, typename P>
+  static __device__ __forceinline__ P* src_any_ptr(const SiteRef& r, P* base) {
+    constexpr int CX = Lat<Q>::c[E][0], CY = Lat<Q>::c[E][1], CZ = Lat<Q>::c[E][2];
+    constexpr uint32_t ES = sizeof(P), BS = (uint32_t)Q * 512u * ES;
+    constexpr bool M = kMortonLanes<std::remove_const_t<P>>;
+    const int lx = (int)(r.w & 7u);
+    const bool bx = CX != 0 && (CX > 0 ? lx == 0 : lx == 7);  // wave-uniform
+    const bool by = lane_cross_y<CY, M>(r.lane);
+    const bool bz = lane_cross_z<CZ, M>(r.lane);
+    constexpr int i000 = 13, i100 = 13 - 9 * CX, i010 = 13 - 3 * CY, i001 = 13 - CZ;
+    constexpr int i110 = 13 - 9 * CX - 3 * CY, i101 = 13 - 9 * CX - CZ;
+    constexpr int i011 = 13 - 3 * CY - CZ, i111 = 13 - 9 * CX - 3 * CY - CZ;
+    const uint32_t uni = ((uint32_t)E * 512u + ((uint32_t)((lx - CX) & 7) << 6)) * ES;
+    const int32_t self = (int32_t)(r.w >> 3);
+    const auto blk = [&](int32_t t) {
+      return __builtin_amdgcn_readfirstlane((uint32_t)(t < 0 ? self : t) * BS + uni);
+    };
+    const uint32_t n00 = blk(bx ? r.tb[i100] : r.tb[i000]);
+    const uint32_t n10 = blk(bx ? r.tb[i110] : r.tb[i010]);
+    const uint32_t n01 = blk(bx ? r.tb[i101] : r.tb[i001]);
+    const uint32_t n11 = blk(bx ? r.tb[i111] : r.tb[i011]);
+    const uint32_t nb = by ? (bz ? n11 : n10) : (bz ? n01 : n00);
+    using C = std::conditional_t<std::is_const<P>::value, const char, char>;
+    return (P*)((C*)base + (nb + lane_shift<CY, CZ, M>(r.lane) * ES));
+  }
   // Streamed store of slot d of the lane's site: a buffer store whose scalar
   // offset carries the wave-uniform part and whose vector offset is the
   // lane's (the same register for every direction), with the non-temporal
@@ -934,6 +962,138 @@ __device__ __forceinline__ double ladd_correction(const DevDomain& D,
   return 2. * L::w[DP] * dot / kCs2;
 }
 
+// Cut distances of links 1..Q-1 from 16-bit fixed-point records (the
+// reduced-precision forms): QS16 = Q - 1 rounded up to a multiple of 8.
+template <int Q> constexpr int kQS16 = (Q - 1 + 7) / 8 * 8;
+template <int Q>
+__device__ __forceinline__ void decode_q16(const uint4 (&v)[kQS16<Q> / 8], float (&q)[kQS16<Q>]) {
+  sfor<0, kQS16<Q> / 8>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    const uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+    sfor<0, 4>([&](auto jj) {
+      constexpr int j = decltype(jj)::value;
+      q[8 * i + 2 * j] = (float)(w[j] & 0xFFFFu) * (1.0f / 65535.0f);
+      q[8 * i + 2 * j + 1] = (float)(w[j] >> 16) * (1.0f / 65535.0f);
+    });
+  });
+}
+
+// BouzidiFirdaousLallemandLink::StreamLink (BouzidiFirdaousLallemand.h:41-66):
+// with q >= 0.5 and a fluid opposite link the slot gets
+// (f*_d + (2q - 1) f*_inv(d)) / (2q); both values are the site's own
+// (opp[e] = f*_e of the site).  Slot e is a candidate (bit e-1 of cand) when
+// the inverse link is a wall link and not an iolet link, and link e itself
+// is not a wall link.
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void bfl_stream_links(const float (&q)[kQS16<Q>], uint32_t cand,
+                                                 const T (&opp)[Q], T (&f)[Q]) {
+  sfor<1, Q>([&](auto ei) {
+    constexpr int e = decltype(ei)::value;
+    constexpr int dp = inv_dir(e);
+    const T qd = (T)(double)q[dp - 1];
+    if (((cand >> (e - 1)) & 1u) && !(qd < T(0.5))) {
+      if constexpr (kShifted<S>)
+        f[e] = mad(T(2.0) * qd - T(1), opp[e], f[e]) * recip(T(2.0) * qd);
+      else
+        f[e] = (f[e] + (T(2.0) * qd - T(1)) * opp[e]) / (T(2.0) * qd);
+    }
+  });
+}
+
+// Iolet links of boundary record `rec`: the Nash ghost equilibrium or the
+// Ladd correction of the bounce-back value.
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void iolet_links(const DevDomain& D, const StepArgs& A, uint32_t rec,
+                                            uint32_t im, T (&f)[Q]) {
+  const int io = D.b_io[rec];
+  const int k = D.io_iolet[io];
+  const int bc = D.io_bc[io];
+  sfor<1, Q>([&](auto ei) {
+    constexpr int e = decltype(ei)::value;
+    constexpr int dp = inv_dir(e);
+    if ((im >> (dp - 1)) & 1u) {
+      if (bc == 0) {
+        double v = nash_value<Q, e>(D, A, io, k);
+        if constexpr (kShifted<S>) v -= Lat<Q>::w[e] * A.rho0;
+        f[e] = (T)v;
+      } else {
+        f[e] = f[e] - (T)ladd_correction<Q, dp>(D, A, io, k);
+      }
+    }
+  });
+}
+
+// BouzidiFirdaousLallemandLink::PostStepLink: links d with a wall and no
+// wall on the inverse link, q < 0.5
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void bfl_post_links(const float (&q)[kQS16<Q>], uint32_t wm,
+                                               T (&f)[Q]) {
+  const uint32_t post = wm & ~pair_swap(wm);
+  sfor<1, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    constexpr int id = inv_dir(d);
+    const T qd = (T)(double)q[d - 1];
+    if (((post >> (d - 1)) & 1u) && qd < T(0.5)) {
+      if constexpr (kShifted<S>)
+        f[id] = mad(T(2.0) * qd, f[id] - f[d], f[d]);
+      else
+        f[id] = T(2.0) * qd * f[id] + (T(1.0) - T(2.0) * qd) * f[d];
+    }
+  });
+}
+
+// Binary16 with 32-bit offsets: every load of a site is issued in one batch
+// after the plane's tables, none waits for the boundary record.  The pulls
+// read x - c_e whether or not link e is cut (an absent neighbour block is
+// replaced by the own block, so the address stays in the state), and lanes
+// with a boundary record also load their own Q-1 values and cut distances;
+// the bounce-back value of a cut link and the BFL value f*_e are then picked
+// from the own values.  The same values and arithmetic as gather's other
+// path: binary16 loads every own value that path picks from anyway.
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void gather_early(const DevDomain& D, const StepArgs& A,
+                                             const S* __restrict__ fin, const SiteRef& r,
+                                             T (&f)[Q]) {
+  using St = Store<S>;
+  using Lay = BlockLayout32;
+  const bool bnd = (D.wave_bnd[r.w] >> r.lane) & 1ull;
+  f[0] = St::ld(Lay::template own_ptr<Q>(r, fin, 0));
+  sfor<1, Q>([&](auto ei) {
+    constexpr int e = decltype(ei)::value;
+    f[e] = St::ld(Lay::template src_any_ptr<Q, e>(r, fin));
+  });
+  T own[Q];
+  uint4 qv[kQS16<Q> / 8];
+  if (bnd) {
+    sfor<1, Q>([&](auto ei) {
+      constexpr int e = decltype(ei)::value;
+      own[e] = St::ld(Lay::template own_ptr<Q>(r, fin, e));
+    });
+    if (A.wall == 1) {
+      const uint4* q8 = reinterpret_cast<const uint4*>(D.b_q16 + (int64_t)r.rec * kQS16<Q>);
+      sfor<0, kQS16<Q> / 8>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        qv[i] = q8[i];
+      });
+    }
+  }
+  if (!bnd) return;
+  sfor<1, Q>([&](auto ei) {
+    constexpr int e = decltype(ei)::value;
+    constexpr int dp = inv_dir(e);
+    if ((r.cut >> (dp - 1)) & 1u) f[e] = own[dp];
+  });
+  const uint32_t rec = r.rec;
+  const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
+  float q[kQS16<Q>];
+  if (A.wall == 1 && wm) {
+    decode_q16<Q>(qv, q);
+    bfl_stream_links<Q, S, T>(q, pair_swap(wm) & ~pair_swap(im) & ~wm, own, f);
+  }
+  if (im) iolet_links<Q, S, T>(D, A, rec, im, f);
+  if (A.wall == 1 && wm) bfl_post_links<Q, S, T>(q, wm, f);
+}
+
 // Post-stream distributions of a site (the reference's fNew after
 // StreamAndCollide and PostStep) from the post-collision state `fin`.
 // All Q loads are issued first: a cut link reads the site's own value of the
@@ -943,6 +1103,10 @@ template <int Q, typename S, typename T, class Lay, bool BULK = false>
 __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
                                        const S* __restrict__ fin, const SiteRef& r,
                                        T (&f)[Q]) {
+  if constexpr (!BULK && sizeof(S) == 2 && std::is_same<Lay, BlockLayout32>::value) {
+    gather_early<Q, S, T>(D, A, fin, r, f);
+    return;
+  }
   using St = Store<S>;
   f[0] = St::ld(own_ptr<Q, Lay>(D, r, fin, 0));
   sfor<1, Q>([&](auto ei) {
@@ -954,23 +1118,18 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
   const uint32_t rec = r.rec;
   const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
   // cut distances of links 1..Q-1 (only the wall links' are read): 16-bit
-  // fixed-point records of QS16 for the shifted forms, binary32 values of
-  // the wall links alone, packed, for binary64 storage
-  constexpr int QS16 = (Q - 1 + 7) / 8 * 8;
-  float q[QS16];
+  // fixed-point records for the shifted forms, binary32 values of the wall
+  // links alone, packed, for binary64 storage
+  float q[kQS16<Q>];
   if (A.wall == 1 && wm) {
     if constexpr (kShifted<S>) {
-      const uint4* q8 = reinterpret_cast<const uint4*>(D.b_q16 + (int64_t)rec * QS16);
-      sfor<0, QS16 / 8>([&](auto ii) {
+      const uint4* q8 = reinterpret_cast<const uint4*>(D.b_q16 + (int64_t)rec * kQS16<Q>);
+      uint4 qv[kQS16<Q> / 8];
+      sfor<0, kQS16<Q> / 8>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
-        const uint4 v = q8[i];
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-        sfor<0, 4>([&](auto jj) {
-          constexpr int j = decltype(jj)::value;
-          q[8 * i + 2 * j] = (float)(w[j] & 0xFFFFu) * (1.0f / 65535.0f);
-          q[8 * i + 2 * j + 1] = (float)(w[j] >> 16) * (1.0f / 65535.0f);
-        });
+        qv[i] = q8[i];
       });
+      decode_q16<Q>(qv, q);
     } else {
       // 4.6 wall links per wall site on the vessel tree: 18 bytes instead
       // of an 80-byte record of every link
@@ -981,16 +1140,11 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
         if ((wm >> k) & 1u) q[k] = qp[__popc(wm & ((1u << k) - 1u))];
       });
     }
-    // BouzidiFirdaousLallemandLink::StreamLink (BouzidiFirdaousLallemand.h:41-66):
-    // with q >= 0.5 and a fluid opposite link the slot gets
-    // (f*_d + (2q - 1) f*_inv(d)) / (2q); both values are the site's own.
     // The links that take the interpolated form need the site's own f*_e.
     // Binary64 and binary32: only those lanes load (the rest are masked
-    // off; measured 0.3-0.6 % faster on the vessel tree).  Binary16: every
-    // lane loads, the unused ones their already-fetched bounce-back slot
-    // (predicated 16-bit loads measured 10 % slower).
-    // Slot e is a candidate when the inverse link is a wall link and not an
-    // iolet link, and link e itself is not a wall link: one mask for all e.
+    // off; measured 0.3-0.6 % faster on the vessel tree).  Binary16 (64-bit
+    // addressing): every lane loads, the unused ones their already-fetched
+    // bounce-back slot (predicated 16-bit loads measured 10 % slower).
     const uint32_t cand = pair_swap(wm) & ~pair_swap(im) & ~wm;
     T opp[Q];
     sfor<1, Q>([&](auto ei) {
@@ -1005,52 +1159,10 @@ __device__ __forceinline__ void gather(const DevDomain& D, const StepArgs& A,
         opp[e] = St::ld(own_pick_ptr<Q, e, dp, Lay>(D, r, fin, link));
       }
     });
-    sfor<1, Q>([&](auto ei) {
-      constexpr int e = decltype(ei)::value;
-      constexpr int dp = inv_dir(e);
-      const T qd = (T)(double)q[dp - 1];
-      if (((cand >> (e - 1)) & 1u) && !(qd < T(0.5))) {
-        if constexpr (kShifted<S>)
-          f[e] = mad(T(2.0) * qd - T(1), opp[e], f[e]) * recip(T(2.0) * qd);
-        else
-          f[e] = (f[e] + (T(2.0) * qd - T(1)) * opp[e]) / (T(2.0) * qd);
-      }
-    });
+    bfl_stream_links<Q, S, T>(q, cand, opp, f);
   }
-  if (im) {
-    const int io = D.b_io[rec];
-    const int k = D.io_iolet[io];
-    const int bc = D.io_bc[io];
-    sfor<1, Q>([&](auto ei) {
-      constexpr int e = decltype(ei)::value;
-      constexpr int dp = inv_dir(e);
-      if ((im >> (dp - 1)) & 1u) {
-        if (bc == 0) {
-          double v = nash_value<Q, e>(D, A, io, k);
-          if constexpr (kShifted<S>) v -= Lat<Q>::w[e] * A.rho0;
-          f[e] = (T)v;
-        } else {
-          f[e] = f[e] - (T)ladd_correction<Q, dp>(D, A, io, k);
-        }
-      }
-    });
-  }
-  if (A.wall == 1 && wm) {
-    // BouzidiFirdaousLallemandLink::PostStepLink: links d with a wall and
-    // no wall on the inverse link, q < 0.5
-    const uint32_t post = wm & ~pair_swap(wm);
-    sfor<1, Q>([&](auto di) {
-      constexpr int d = decltype(di)::value;
-      constexpr int id = inv_dir(d);
-      const T qd = (T)(double)q[d - 1];
-      if (((post >> (d - 1)) & 1u) && qd < T(0.5)) {
-        if constexpr (kShifted<S>)
-          f[id] = mad(T(2.0) * qd, f[id] - f[d], f[d]);
-        else
-          f[id] = T(2.0) * qd * f[id] + (T(1.0) - T(2.0) * qd) * f[d];
-      }
-    });
-  }
+  if (im) iolet_links<Q, S, T>(D, A, rec, im, f);
+  if (A.wall == 1 && wm) bfl_post_links<Q, S, T>(q, wm, f);
 }
 
 template <int Q, typename T, bool BULK = false>
