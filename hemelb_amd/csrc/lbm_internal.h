@@ -92,10 +92,10 @@ struct StepArgs {
   // it) and the workgroup-to-XCD remap (0: hardware round-robin order; 1:
   // one contiguous range per XCD; K = 2^k: runs of K workgroups per XCD,
   // the eight XCDs on adjacent runs).  Default (set per storage by the
-  // solver): 128 threads in short runs of workgroups per XCD (2 for
-  // binary64, 8 for binary32, the hardware order for binary16), so that the
-  // whole chip sweeps one region of the domain at a time; one range per XCD
-  // measured 4-16 % slower on the vessel tree.
+  // solver): short runs of workgroups per XCD (128 threads in runs of 2 for
+  // binary64 and of 8 for binary32, 64 threads in runs of 16 for
+  // binary16), so that the whole chip sweeps one region of the domain at a
+  // time; one range per XCD measured 4-16 % slower on the vessel tree.
   uint32_t threads = 128;
   uint32_t planes = 1;
   uint32_t groups = 0;

@@ -1070,13 +1070,14 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
   A.rho0 = P.storage == HLB_STORE_FP64 ? 0.0 : P.rho0;
   // MRT relaxation rate of the viscous moments (DHumieres*MRTBasis.cc)
   A.mrt_s_tau = 1.0 / P.tau;
-  // Default launch shape: 128-thread workgroups; in the workgroup-to-XCD
-  // order that measured fastest on the vessel tree for each storage (two
-  // sweeps of every shape): runs of 2 workgroups (one half block) per XCD
-  // for binary64 (0.7 % ahead of runs of 8), runs of 8 for binary32, the
-  // hardware order for binary16 (3 % ahead of runs of 8).
-  A.threads = 128;
-  A.xcd_remap = P.storage == HLB_STORE_FP64 ? 2u : P.storage == HLB_STORE_FP32 ? 8u : 0u;
+  // Default launch shape: the workgroup size and workgroup-to-XCD order
+  // that measured fastest on the vessel tree for each storage (two sweeps
+  // of every shape): 128-thread workgroups in runs of 2 (one half block)
+  // per XCD for binary64 (0.7 % ahead of runs of 8) and in runs of 8 (two
+  // blocks) for binary32; 64-thread workgroups in runs of 16 (two blocks)
+  // for binary16 (2 % ahead of 128 threads in the hardware order).
+  A.threads = P.storage == HLB_STORE_FP16 ? 64u : 128u;
+  A.xcd_remap = P.storage == HLB_STORE_FP64 ? 2u : P.storage == HLB_STORE_FP32 ? 8u : 16u;
   s->initialised = false;
   *out = s.release();
   return HLB_OK;
