@@ -157,7 +157,9 @@ def shape_sweep(args) -> int:
 
     w = WL.vessel_tree("LBGK")
     n = w.domain.n
-    shapes = [(t, r) for t in (128, 256) for r in (0, 1, 2, 8, 32)] + [(64, 0), (64, 8)]
+    shapes = ([(128, r) for r in (0, 1, 2, 4, 8, 16, 32, 64)] +
+              [(256, r) for r in (0, 1, 2, 8, 32)] +
+              [(64, r) for r in (0, 4, 8, 16, 32, 64, 128, 256)])
     for storage in ("fp64", "fp32", "fp16"):
         lbm = w.solver(storage=storage)
         lbm.step(5)

@@ -1075,7 +1075,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
   // of every shape): 128-thread workgroups in runs of 2 (one half block)
   // per XCD for binary64 (0.7 % ahead of runs of 8) and in runs of 8 (two
   // blocks) for binary32; 64-thread workgroups in runs of 16 (two blocks)
-  // for binary16 (2 % ahead of 128 threads in the hardware order).
+  // for binary16 (1-2 % ahead of 128 threads in the hardware order).
   A.threads = P.storage == HLB_STORE_FP16 ? 64u : 128u;
   A.xcd_remap = P.storage == HLB_STORE_FP64 ? 2u : P.storage == HLB_STORE_FP32 ? 8u : 16u;
   s->initialised = false;
