@@ -272,6 +272,26 @@ __host__ __device__ constexpr double mrt_diag(int k) {
   return acc;
 }
 
+// Parity of MRT moment k under inversion of the directions, which come in
+// opposite pairs (d, d+1), d odd: 1 when row k of the basis takes the same
+// value on both directions of every pair, -1 when opposite values (and 0 on
+// the rest direction), 0 when neither.
+template <int Q>
+__host__ __device__ constexpr int mrt_parity(int k) {
+  bool even = true, odd = Lat<Q>::M[k][0] == 0;
+  for (int d = 1; d < Q; d += 2) {
+    if (Lat<Q>::M[k][d] != Lat<Q>::M[k][d + 1]) even = false;
+    if (Lat<Q>::M[k][d] != -Lat<Q>::M[k][d + 1]) odd = false;
+  }
+  return even ? 1 : (odd ? -1 : 0);
+}
+template <int Q>
+__host__ __device__ constexpr bool mrt_rows_have_parity() {
+  for (int k = 0; k < Lat<Q>::nmom; ++k)
+    if (mrt_parity<Q>(k) == 0) return false;
+  return true;
+}
+
 // Sums over the binary32-rounded weights (exact, in binary64): sum_i w_i and
 // sum_i w_i c_ix^2 (the same for y and z on both lattices).
 template <int Q>
@@ -399,6 +419,86 @@ __device__ __forceinline__ Hydro<Q, T> collide_lbgk_shifted(T (&f)[Q], const Ste
   return h;
 }
 
+// MRT relaxation of the shifted storage forms in binary32 (these follow no
+// reference rounding): the same m_neq = M f_neq and f* = f - sum_k (S_k N_kd)
+// m_neq_k as the binary64 form, with the rows of the d'Humieres basis
+// grouped by their parity under inversion.  An even row needs the rest value
+// and the pair sums of f_neq, an odd row the pair differences; the
+// relaxation of pair (d, d+1) is E + O on d and E - O on d+1, with E the
+// even moments' part and O the odd moments'.  About half the operations of
+// the row-by-column form.
+template <int Q, typename T>
+__device__ __forceinline__ void mrt_relax_pairs(const T (&fneq)[Q], const StepArgs& a,
+                                                T (&f)[Q]) {
+  using L = Lat<Q>;
+  static_assert(pairs_opposite<Q>() && mrt_rows_have_parity<Q>(),
+                "MRT pair form: opposite direction pairs and rows of definite parity");
+  constexpr int NM = L::nmom;
+  T sg[Q], df[Q];  // pair sums and differences, at the pair's first direction
+  sfor<1, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    if constexpr (d % 2 == 1) {
+      sg[d] = fneq[d] + fneq[d + 1];
+      df[d] = fneq[d] - fneq[d + 1];
+    }
+  });
+  T mneq[NM];
+  sfor<0, NM>([&](auto ki) {
+    constexpr int k = decltype(ki)::value;
+    constexpr bool even = mrt_parity<Q>(k) == 1;
+    T acc = T(0.);
+    if constexpr (even && L::M[k][0] != 0) acc = T(L::M[k][0]) * fneq[0];
+    sfor<1, Q>([&](auto di) {
+      constexpr int d = decltype(di)::value;
+      constexpr int m = L::M[k][d];
+      if constexpr (d % 2 == 1 && m != 0) {
+        const T v = even ? sg[d] : df[d];
+        if constexpr (m == 1) acc = acc + v;
+        else if constexpr (m == -1) acc = acc - v;
+        else acc = mad(T(m), v, acc);
+      }
+    });
+    mneq[k] = acc;
+  });
+  // S_k N_kd as in the binary64 form (the 1/tau moments from the runtime rate)
+  const double s_tau = a.mrt_s_tau;
+  const auto coef = [&](auto ki, auto di) {
+    constexpr int k = decltype(ki)::value, d = decltype(di)::value;
+    constexpr double N = (double)L::M[k][d] / mrt_diag<Q>(k);
+    constexpr double rate = mrt_rate(Q, k);
+    if constexpr (rate != 0.0) {
+      constexpr double cc = rate * N;
+      return T(cc);
+    } else {
+      return T(s_tau * N);
+    }
+  };
+  {
+    T e0 = T(0.);
+    sfor<0, NM>([&](auto ki) {
+      constexpr int k = decltype(ki)::value;
+      if constexpr (mrt_parity<Q>(k) == 1 && L::M[k][0] != 0)
+        e0 = mad(coef(ki, std::integral_constant<int, 0>{}), mneq[k], e0);
+    });
+    f[0] = f[0] - e0;
+  }
+  sfor<1, Q>([&](auto di) {
+    constexpr int d = decltype(di)::value;
+    if constexpr (d % 2 == 1) {
+      T ev = T(0.), od = T(0.);
+      sfor<0, NM>([&](auto ki) {
+        constexpr int k = decltype(ki)::value;
+        if constexpr (L::M[k][d] != 0) {
+          if constexpr (mrt_parity<Q>(k) == 1) ev = mad(coef(ki, di), mneq[k], ev);
+          else od = mad(coef(ki, di), mneq[k], od);
+        }
+      });
+      f[d] = f[d] - (ev + od);
+      f[d + 1] = f[d + 1] - (ev - od);
+    }
+  });
+}
+
 // Collide in place: f (post-stream, possibly shifted) -> f* ; returns the
 // moments the streamers of the next step need.  Binary64 LBGK and MRT in
 // the reference's evaluation order, MRT of the shifted forms below; LBGK of
@@ -437,6 +537,8 @@ __device__ __forceinline__ Hydro<Q, T> collide(T (&f)[Q], const StepArgs& a) {
         constexpr int d = decltype(di)::value;
         f[d] = mad(fneq[d], omega, f[d]);
       });
+    } else if constexpr (SHIFT) {
+      mrt_relax_pairs<Q, T>(fneq, a, f);
     } else {
       // MRT: m_neq = M f_neq (MRT.h:123-135); f* = f - sum_k (S_k N_kd) m_neq_k
       // with N = M / diag(M M^T) (MRT.h:139-152).  S_k N_kd is a compile-time
