@@ -30,6 +30,7 @@ CASES = [
      lambda: K.cylinder_case(8.0, 40, 19, "LBGK", "BFL", "womersley", tau=0.7), 600),
     ("cyl15-lbgk-sbb-nash", lambda: K.cylinder_case(8.0, 40, 15, "LBGK", "SBB", "cosine"), 600),
     ("tree19-mrt-bfl-ladd", lambda: K.tree_case(3, 8.0, seed=2, kernel="MRT"), 400),
+    ("cyl15-mrt-bfl-nash", lambda: K.cylinder_case(8.0, 40, 15, "MRT", "BFL", "cosine"), 600),
 ]
 
 # relative L2 velocity error bounds against binary64
