@@ -30,6 +30,11 @@ struct DevDomain {
   const uint64_t* wave_bnd = nullptr;    // [nwaves] lane has a boundary record
   const uint32_t* wave_rec = nullptr;    // [nwaves] first boundary record
   const uint32_t* wave_cbase = nullptr;  // [nwaves] compact index of first fluid lane
+  // [nwaves] OR over the plane's boundary sites of their BFL candidate links
+  // (bits 0-31: inverse link a wall link, not an iolet link, link itself no
+  // wall link) and of their post-step links (bits 32-63: wall link whose
+  // inverse is no wall link)
+  const uint64_t* wave_links = nullptr;
   // Block layouts: the planes holding fluid split into those without any
   // boundary site and those with one (Morton order within each list).
   const uint32_t* bulk_waves = nullptr;

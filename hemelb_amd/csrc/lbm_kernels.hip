@@ -1085,13 +1085,16 @@ __device__ __forceinline__ void decode_q16(const uint4 (&v)[kQS16<Q> / 8], float
 // (f*_d + (2q - 1) f*_inv(d)) / (2q); both values are the site's own
 // (opp[e] = f*_e of the site).  Slot e is a candidate (bit e-1 of cand) when
 // the inverse link is a wall link and not an iolet link, and link e itself
-// is not a wall link.
+// is not a wall link.  A link outside `plane` (the union of cand over the
+// wave's lanes, a scalar) is passed over by one scalar branch.
 template <int Q, typename S, typename T>
 __device__ __forceinline__ void bfl_stream_links(const float (&q)[kQS16<Q>], uint32_t cand,
-                                                 const T (&opp)[Q], T (&f)[Q]) {
+                                                 const T (&opp)[Q], T (&f)[Q],
+                                                 uint32_t plane = ~0u) {
   sfor<1, Q>([&](auto ei) {
     constexpr int e = decltype(ei)::value;
     constexpr int dp = inv_dir(e);
+    if (!((plane >> (e - 1)) & 1u)) return;
     const T qd = (T)(double)q[dp - 1];
     if (((cand >> (e - 1)) & 1u) && !(qd < T(0.5))) {
       if constexpr (kShifted<S>)
@@ -1126,14 +1129,16 @@ __device__ __forceinline__ void iolet_links(const DevDomain& D, const StepArgs& 
 }
 
 // BouzidiFirdaousLallemandLink::PostStepLink: links d with a wall and no
-// wall on the inverse link, q < 0.5
+// wall on the inverse link, q < 0.5 (`plane`: the union of those links over
+// the wave's lanes, as for bfl_stream_links)
 template <int Q, typename S, typename T>
 __device__ __forceinline__ void bfl_post_links(const float (&q)[kQS16<Q>], uint32_t wm,
-                                               T (&f)[Q]) {
+                                               T (&f)[Q], uint32_t plane = ~0u) {
   const uint32_t post = wm & ~pair_swap(wm);
   sfor<1, Q>([&](auto di) {
     constexpr int d = decltype(di)::value;
     constexpr int id = inv_dir(d);
+    if (!((plane >> (d - 1)) & 1u)) return;
     const T qd = (T)(double)q[d - 1];
     if (((post >> (d - 1)) & 1u) && qd < T(0.5)) {
       if constexpr (kShifted<S>)
@@ -1187,13 +1192,15 @@ __device__ __forceinline__ void gather_early(const DevDomain& D, const StepArgs&
   });
   const uint32_t rec = r.rec;
   const uint32_t wm = D.b_wall[rec], im = D.b_iolet[rec];
+  // the plane's BFL links: formulas of links no lane has are skipped
+  const uint64_t wl = D.wave_links[r.w];
   float q[kQS16<Q>];
   if (A.wall == 1 && wm) {
     decode_q16<Q>(qv, q);
-    bfl_stream_links<Q, S, T>(q, pair_swap(wm) & ~pair_swap(im) & ~wm, own, f);
+    bfl_stream_links<Q, S, T>(q, pair_swap(wm) & ~pair_swap(im) & ~wm, own, f, (uint32_t)wl);
   }
   if (im) iolet_links<Q, S, T>(D, A, rec, im, f);
-  if (A.wall == 1 && wm) bfl_post_links<Q, S, T>(q, wm, f);
+  if (A.wall == 1 && wm) bfl_post_links<Q, S, T>(q, wm, f, (uint32_t)(wl >> 32));
 }
 
 // Post-stream distributions of a site (the reference's fNew after

@@ -273,7 +273,7 @@ struct hlb_solver {
   DevBuf<uint16_t> b_q16;
   DevBuf<double> io_stash, io_ladd, iolet_n, weights;
   DevBuf<int32_t> bnbr;
-  DevBuf<uint64_t> wave_fluid, wave_bnd;
+  DevBuf<uint64_t> wave_fluid, wave_bnd, wave_links;
   DevBuf<uint32_t> wave_rec, wave_cbase, bulk_waves, bnd_waves;
   std::vector<int64_t> slot_of;   // compact -> state index (slot or compact)
   DevBuf<int64_t> d_perm;
@@ -343,7 +343,7 @@ struct hlb_solver {
          b_io.bytes() + io_iolet.bytes() + io_bc.bytes() + io_site.bytes() +
          iolet_kind.bytes() + b_q.bytes() + b_qoff.bytes() + b_q16.bytes() + iolet_nf.bytes() + io_stash.bytes() +
          io_ladd.bytes() + iolet_n.bytes() + weights.bytes() + d_perm.bytes() +
-         bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() +
+         bnbr.bytes() + wave_fluid.bytes() + wave_bnd.bytes() + wave_links.bytes() +
          wave_rec.bytes() +
          wave_cbase.bytes() + bulk_waves.bytes() + bnd_waves.bytes() +
          s_soa.bytes() + s_aos.bytes() + rho.bytes() + u.bytes() + u_snap.bytes();
@@ -815,7 +815,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
 
   // Block layout tables: neighbour blocks and per-wave masks.
   std::vector<int32_t> h_bnbr;
-  std::vector<uint64_t> h_wf, h_wb;
+  std::vector<uint64_t> h_wf, h_wb, h_wl;
   std::vector<uint32_t> h_bulkw, h_bndw;
   std::vector<uint32_t> h_wr, h_wc;
   if (blocks_layout) {
@@ -852,6 +852,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
     if (nwaves >= (int64_t)0xFFFFFFFFll) return fail(HLB_E_INVALID, "domain too large");
     h_wf.assign(nwaves, 0);
     h_wb.assign(nwaves, 0);
+    h_wl.assign(nwaves, 0);
     h_wr.assign(nwaves, 0);
     h_wc.assign(nwaves, 0);
     uint32_t rec = 0;
@@ -865,8 +866,16 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
         const int lane = (int)(bslot(o) & 63);
         const int64_t i = s->perm[o];
         h_wf[w] |= 1ull << lane;
-        if (dm->wall_mask[i] | dm->iolet_mask[i]) {
+        const uint32_t wm = dm->wall_mask[i], im = dm->iolet_mask[i];
+        if (wm | im) {
           h_wb[w] |= 1ull << lane;
+          // the plane's BFL links (the gather's masks; directions are
+          // numbered in inverse pairs (1,2), (3,4), ...)
+          const auto swap = [](uint32_t v) {
+            return ((v & 0x55555555u) << 1) | ((v >> 1) & 0x55555555u);
+          };
+          h_wl[w] |= (uint64_t)(swap(wm) & ~swap(im) & ~wm) |
+                     ((uint64_t)(wm & ~swap(wm)) << 32);
           ++rec;
         }
         ++o;
@@ -976,6 +985,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
   HIP_OR_FAIL(s->bnbr.upload(h_bnbr));
   HIP_OR_FAIL(s->wave_fluid.upload(h_wf));
   HIP_OR_FAIL(s->wave_bnd.upload(h_wb));
+  HIP_OR_FAIL(s->wave_links.upload(h_wl));
   HIP_OR_FAIL(s->wave_rec.upload(h_wr));
   HIP_OR_FAIL(s->wave_cbase.upload(h_wc));
   HIP_OR_FAIL(s->bulk_waves.upload(h_bulkw));
@@ -1029,6 +1039,7 @@ static int create_impl(const hlb_params* params, const hlb_domain* dm, const hlb
   D.bnbr = s->bnbr.p;
   D.wave_fluid = s->wave_fluid.p;
   D.wave_bnd = s->wave_bnd.p;
+  D.wave_links = s->wave_links.p;
   D.wave_rec = s->wave_rec.p;
   D.wave_cbase = s->wave_cbase.p;
   if (blocks_layout) {
