@@ -53,9 +53,19 @@ def test_reduced_storage_error_against_binary64(label, make, steps, storage):
     assert err["wall_shear_stress"]["l2_rel"] <= 10 * BOUND[storage]
 
 
+LAYOUT_CASES = [
+    ("tree19-lbgk-bfl-ladd", lambda: K.tree_case(3, 6.0, seed=1)),
+    ("cyl15-mrt-bfl-nash", lambda: K.cylinder_case(6.0, 24, 15, "MRT", "BFL", "cosine")),
+]
+
+
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
-def test_reduced_storage_layouts_agree_bitwise(storage):
-    case = K.tree_case(3, 6.0, seed=1)
+@pytest.mark.parametrize("label,make", LAYOUT_CASES, ids=[c[0] for c in LAYOUT_CASES])
+def test_reduced_storage_layouts_agree_bitwise(label, make, storage):
+    """The block layouts (binary16 with 32-bit offsets issues all of a
+    site's loads in one batch) against the pull table, bit for bit."""
+    case = make()
+    print(f"[precision] layouts {label} {storage}", flush=True)
     a, _ = _fields(case, storage, 30, "blocks")
     for layout in ("pull_table", "compact_blocks"):
         b, _ = _fields(case, storage, 30, layout)
