@@ -148,9 +148,9 @@ __device__ __forceinline__ float mad(float a, float b, float c) {
 }
 __device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
 
-// 1/x.  Binary32: the hardware reciprocal (1 ulp), binary64: division.
+// 1/x in binary32: the hardware reciprocal (1 ulp).  (Binary64 formulas
+// divide.)
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 
 // ---------------------------------------------------------------------------
 // Storage: binary64 holds f; binary32/binary16 hold f - w_i*rho0.
