@@ -12,8 +12,10 @@ Prints one JSON line with:
 * MLUPS and the dominant kernel's HBM roofline, beside the rate of the
   library's device copy of the same state arrays (hlb_copy_ceiling);
 * the rates of the reduced-precision storage forms on the same workload;
-* the wall time from rest to the periodic state (binary64 and the FP32
-  form);
+* the wall time from rest to the periodic state: binary64, the FP32 form,
+  and binary64 with the transient run in binary16 (its state handed over
+  exactly once its own cycle-to-cycle change is below 1e-2 or stops
+  halving);
 * the CPU oracle's rate (all host threads) on a bounded sample of the
   workload.
 
@@ -85,26 +87,48 @@ def cpu_baseline(dom, tau, inlets, outlets, steps, seconds=10.0):
 
 
 def time_to_periodic(work, storage: str, layout: str, tol: float = 1e-4,
-                     max_cycles: int = 12) -> dict:
+                     max_cycles: int = 12, keep_velocity: bool = False):
     """Wall time of the plain run from rest until the cycle-to-cycle change
-    max |u(t) - u(t - T)| / max |u| drops below tol (the periodic-state
-    monitor), one cycle of the inlet period per check."""
+    drops below tol, one cycle of the inlet period per check.  With
+    keep_velocity, also the final velocity field (host array)."""
+    from hemelb_amd import extraction as X
+
     lbm = work.solver(storage=storage, layout=layout)
     lbm.synchronize()
     period = int(round(work.period))
     t0 = time.perf_counter()
-    lbm.snapshot_velocity()
-    change, cycles = float("inf"), 0
-    while cycles < max_cycles and not change < tol:
-        lbm.step(period)
-        cycles += 1
-        umax = lbm.monitor()["velocity_max"]
-        change = lbm.velocity_change(max(umax, 1e-30))
-        lbm.snapshot_velocity()
+    cycles, history = X.cycles_to_periodic(lbm, period, tol, max_cycles)
     lbm.synchronize()
     wall = time.perf_counter() - t0
-    return {"cycles": cycles if change < tol else None, "steps": cycles * period,
-            "period": period, "wall_s": round(wall, 3), "last_change": change, "tol": tol}
+    res = {"cycles": cycles, "steps": lbm.GetTimeStep() - 1, "period": period,
+           "wall_s": round(wall, 3), "last_change": history[-1], "tol": tol,
+           "changes": [float(f"{c:.3g}") for c in history]}
+    return (res, lbm.macroscopic()[1]) if keep_velocity else res
+
+
+def time_to_periodic_continued(work, first: str, final: str, layout: str,
+                               tol: float = 1e-4, tol_first: float = 1e-2,
+                               max_cycles: int = 12):
+    """Wall time from rest to the periodic state of the `final` storage form
+    with the transient run in the faster `first` form: `first` runs from
+    rest until its cycle-to-cycle change is below tol_first or stops
+    halving per cycle; its distributions (exact in binary64) then start
+    `final` at the same time step, which runs until its own change is below
+    tol.  Returns the timings and the final velocity field (host array)."""
+    from hemelb_amd import extraction as X
+
+    a = work.solver(storage=first, layout=layout)
+    b = work.solver(storage=final, layout=layout)
+    period = int(round(work.period))
+    r = X.periodic_state_continued(a, b, period, tol, tol_first, max_cycles)
+    res = {"first": first, "cycles_first": r["cycles_first"], "tol_first": tol_first,
+           "wall_first_s": round(r["seconds_first"], 3),
+           "handover_s": round(r["handover_seconds"], 3), "cycles": r["cycles"],
+           "steps": b.GetTimeStep() - 1, "period": period, "wall_s": round(r["seconds"], 3),
+           "last_change": r["changes"][-1], "tol": tol,
+           "changes_first": [float(f"{c:.3g}") for c in r["changes_first"]],
+           "changes": [float(f"{c:.3g}") for c in r["changes"]]}
+    return res, b.macroscopic()[1]
 
 
 def timed_rate(lbm, n_sites, steps, warmup):
@@ -367,9 +391,34 @@ def main() -> int:
     periodic = None
     if not args.no_periodic:
         periodic = {}
+        u_plain = None
         for st in (args.storage, "fp32") if args.storage != "fp32" else ("fp32",):
-            periodic[st] = time_to_periodic(work, st, args.layout)
+            if st == args.storage:
+                periodic[st], u_plain = time_to_periodic(work, st, args.layout,
+                                                         keep_velocity=True)
+            else:
+                periodic[st] = time_to_periodic(work, st, args.layout)
             log(f"[bench] {st} from rest to the periodic state: {periodic[st]}")
+        if args.storage != "fp16":
+            # the transient in binary16, the periodic state in the headline form
+            key = f"fp16>{args.storage}"
+            try:
+                res, u_cont = time_to_periodic_continued(work, "fp16", args.storage,
+                                                         args.layout)
+                if (u_plain is not None and res["cycles"] is not None and
+                        periodic[args.storage]["cycles"] is not None and
+                        (res["steps"] - periodic[args.storage]["steps"]) % res["period"] == 0):
+                    # same phase: the two periodic fields, relative to max |u|
+                    import numpy as np
+                    scale = max(float(np.abs(u_plain).max()), 1e-30)
+                    res["velocity_diff_vs_plain"] = (float(np.abs(u_cont - u_plain).max()) /
+                                                     scale)
+                del u_cont
+            except Exception as e:  # reported in the line; the headline stands
+                res = {"error": f"{type(e).__name__}: {e}"}
+            periodic[key] = res
+            log(f"[bench] {key} from rest to the periodic state: {res}")
+        del u_plain
     cpu = None
     if not args.no_cpu_baseline:
         c_mlups, c_s, c_steps, c_threads = cpu_baseline(dom, tau, inlets, outlets,

@@ -113,6 +113,56 @@ class PeriodicStateMonitor:
                 "last_change": self.history[-1] if self.history else float("nan")}
 
 
+def cycles_to_periodic(lbm: LBM, period: int, tolerance: float = 1e-4,
+                       max_cycles: int = 12, stop_on_plateau: bool = False):
+    """Runs `lbm` one inlet cycle at a time until the cycle-to-cycle change
+    max |u(t) - u(t - T)| / max |u| drops below `tolerance`, or, with
+    stop_on_plateau, until a cycle no longer halves it (the rounding floor
+    of a reduced-precision form).  Returns the cycles run when it stopped
+    for either reason (None after max_cycles) and every cycle's change."""
+    lbm.snapshot_velocity()
+    history, cycles, done = [], 0, False
+    while cycles < max_cycles and not done:
+        lbm.step(int(period))
+        cycles += 1
+        umax = lbm.monitor()["velocity_max"]
+        history.append(lbm.velocity_change(max(umax, 1e-30)))
+        lbm.snapshot_velocity()
+        done = history[-1] < tolerance or (stop_on_plateau and len(history) > 1 and
+                                           history[-1] > 0.5 * history[-2])
+    return (cycles if done else None), history
+
+
+def periodic_state_continued(first: LBM, final: LBM, period: int, tolerance: float = 1e-4,
+                             first_tolerance: float = 1e-2, max_cycles: int = 12) -> dict:
+    """The periodic state of `final` with the transient run in `first`, a
+    faster (reduced-precision) solver of the same domain and iolets.
+
+    `first` runs from its current state until its cycle-to-cycle change is
+    below `first_tolerance` or stops halving; its distributions, exact in
+    binary64, and its time step then start `final`, which runs until its
+    own change is below `tolerance`.  `final` ends in its own periodic state
+    (to `tolerance`), at a time step that is a whole number of cycles from
+    `first`'s start.  Returns the cycles, changes and wall times."""
+    first.synchronize()
+    final.synchronize()
+    t0 = time.perf_counter()
+    c1, h1 = cycles_to_periodic(first, period, first_tolerance, max_cycles,
+                                stop_on_plateau=True)
+    first.synchronize()
+    t1 = time.perf_counter()
+    final.set_distributions(first.get_distributions())
+    final.SetTimeStep(first.GetTimeStep())
+    final.synchronize()
+    t2 = time.perf_counter()
+    c2, h2 = cycles_to_periodic(final, period, tolerance, max_cycles)
+    final.synchronize()
+    t3 = time.perf_counter()
+    return {"cycles_first": c1, "changes_first": h1, "seconds_first": t1 - t0,
+            "handover_seconds": t2 - t1, "cycles": c2, "changes": h2,
+            "seconds": t3 - t0, "converged": c2 is not None}
+
+
 # ---------------------------------------------------------------------------
 # Extraction (.xtr) files and checkpoints
 # ---------------------------------------------------------------------------
