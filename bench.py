@@ -13,9 +13,9 @@ Prints one JSON line with:
   library's device copy of the same state arrays (hlb_copy_ceiling);
 * the rates of the reduced-precision storage forms on the same workload;
 * the wall time from rest to the periodic state: binary64, the FP32 form,
-  and binary64 with the transient run in binary16 (its state handed over
-  exactly once its own cycle-to-cycle change is below 1e-2 or stops
-  halving);
+  and binary64 with the transient run in binary16 and one binary32 cycle
+  (each state handed over exactly once the cycle-to-cycle change is below
+  1e-2 or stops halving);
 * the CPU oracle's rate (all host threads) on a bounded sample of the
   workload.
 
@@ -106,29 +106,30 @@ def time_to_periodic(work, storage: str, layout: str, tol: float = 1e-4,
     return (res, lbm.macroscopic()[1]) if keep_velocity else res
 
 
-def time_to_periodic_continued(work, first: str, final: str, layout: str,
-                               tol: float = 1e-4, tol_first: float = 1e-2,
-                               max_cycles: int = 12):
-    """Wall time from rest to the periodic state of the `final` storage form
-    with the transient run in the faster `first` form: `first` runs from
-    rest until its cycle-to-cycle change is below tol_first or stops
-    halving per cycle; its distributions (exact in binary64) then start
-    `final` at the same time step, which runs until its own change is below
-    tol.  Returns the timings and the final velocity field (host array)."""
+def time_to_periodic_staged(work, chain, layout: str, tol: float = 1e-4,
+                            stage_tol: float = 1e-2, max_cycles: int = 12):
+    """Wall time from rest to the periodic state of the last storage form of
+    `chain` with the transient run in the earlier, faster forms
+    (hemelb_amd.extraction.periodic_state_staged): each earlier form runs
+    until its cycle-to-cycle change is below stage_tol or stops halving and
+    hands its state (exact in binary64) and time step to the next; the last
+    runs until its own change is below tol.  Returns the timings and the
+    final velocity field (host array)."""
     from hemelb_amd import extraction as X
 
-    a = work.solver(storage=first, layout=layout)
-    b = work.solver(storage=final, layout=layout)
+    stages = [work.solver(storage=st, layout=layout) for st in chain]
     period = int(round(work.period))
-    r = X.periodic_state_continued(a, b, period, tol, tol_first, max_cycles)
-    res = {"first": first, "cycles_first": r["cycles_first"], "tol_first": tol_first,
-           "wall_first_s": round(r["seconds_first"], 3),
-           "handover_s": round(r["handover_seconds"], 3), "cycles": r["cycles"],
-           "steps": b.GetTimeStep() - 1, "period": period, "wall_s": round(r["seconds"], 3),
-           "last_change": r["changes"][-1], "tol": tol,
-           "changes_first": [float(f"{c:.3g}") for c in r["changes_first"]],
-           "changes": [float(f"{c:.3g}") for c in r["changes"]]}
-    return res, b.macroscopic()[1]
+    r = X.periodic_state_staged(stages, period, tol, stage_tol, max_cycles)
+    last = stages[-1]
+    res = {"chain": list(chain), "stage_tol": stage_tol,
+           "stage_cycles": [x["cycles"] for x in r["stages"]],
+           "stage_wall_s": [round(x["seconds"], 3) for x in r["stages"]],
+           "stage_changes": [[float(f"{c:.3g}") for c in x["changes"]] for x in r["stages"]],
+           "handover_s": round(r["handover_seconds"], 3),
+           "cycles": r["stages"][-1]["cycles"], "steps": last.GetTimeStep() - 1,
+           "period": period, "wall_s": round(r["seconds"], 3),
+           "last_change": r["stages"][-1]["changes"][-1], "tol": tol}
+    return res, last.macroscopic()[1]
 
 
 def timed_rate(lbm, n_sites, steps, warmup):
@@ -400,11 +401,13 @@ def main() -> int:
                 periodic[st] = time_to_periodic(work, st, args.layout)
             log(f"[bench] {st} from rest to the periodic state: {periodic[st]}")
         if args.storage != "fp16":
-            # the transient in binary16, the periodic state in the headline form
-            key = f"fp16>{args.storage}"
+            # the transient in binary16, one binary32 cycle to absorb the
+            # handover, the periodic state in the headline form
+            chain = ["fp16", "fp32", args.storage] if args.storage == "fp64" else \
+                ["fp16", args.storage]
+            key = ">".join(chain)
             try:
-                res, u_cont = time_to_periodic_continued(work, "fp16", args.storage,
-                                                         args.layout)
+                res, u_cont = time_to_periodic_staged(work, chain, args.layout)
                 if (u_plain is not None and res["cycles"] is not None and
                         periodic[args.storage]["cycles"] is not None and
                         (res["steps"] - periodic[args.storage]["steps"]) % res["period"] == 0):

@@ -133,34 +133,53 @@ def cycles_to_periodic(lbm: LBM, period: int, tolerance: float = 1e-4,
     return (cycles if done else None), history
 
 
+def periodic_state_staged(stages: list, period: int, tolerance: float = 1e-4,
+                          stage_tolerance: float = 1e-2, max_cycles: int = 12) -> dict:
+    """The periodic state of the last solver of `stages` with the transient
+    run in the earlier, faster (reduced-precision) ones, all of the same
+    domain and iolets.
+
+    Each earlier stage runs from its current state until its cycle-to-cycle
+    change is below `stage_tolerance` or stops halving.  Its distributions,
+    exact in binary64, and its time step then start the next stage.  The
+    last stage runs until its own change is below `tolerance`; it ends in
+    its own periodic state (to `tolerance`) at a whole number of cycles from
+    the first stage's start.  A middle stage absorbs the error that the
+    handover from a coarser form brings into the first cycle of the next,
+    so that the last stage can meet the tolerance in one cycle.  Returns the
+    cycles, changes and wall times of every stage."""
+    for lbm in stages:
+        lbm.synchronize()
+    t0 = time.perf_counter()
+    out, handover = [], 0.0
+    for k, lbm in enumerate(stages):
+        last = k == len(stages) - 1
+        if k:
+            t = time.perf_counter()
+            lbm.set_distributions(stages[k - 1].get_distributions())
+            lbm.SetTimeStep(stages[k - 1].GetTimeStep())
+            lbm.synchronize()
+            handover += time.perf_counter() - t
+        t = time.perf_counter()
+        cycles, history = cycles_to_periodic(lbm, period, tolerance if last else stage_tolerance,
+                                             max_cycles, stop_on_plateau=not last)
+        lbm.synchronize()
+        out.append({"cycles": cycles, "changes": history, "seconds": time.perf_counter() - t})
+    return {"stages": out, "handover_seconds": handover, "seconds": time.perf_counter() - t0,
+            "converged": out[-1]["cycles"] is not None}
+
+
 def periodic_state_continued(first: LBM, final: LBM, period: int, tolerance: float = 1e-4,
                              first_tolerance: float = 1e-2, max_cycles: int = 12) -> dict:
-    """The periodic state of `final` with the transient run in `first`, a
-    faster (reduced-precision) solver of the same domain and iolets.
-
-    `first` runs from its current state until its cycle-to-cycle change is
-    below `first_tolerance` or stops halving; its distributions, exact in
-    binary64, and its time step then start `final`, which runs until its
-    own change is below `tolerance`.  `final` ends in its own periodic state
-    (to `tolerance`), at a time step that is a whole number of cycles from
-    `first`'s start.  Returns the cycles, changes and wall times."""
-    first.synchronize()
-    final.synchronize()
-    t0 = time.perf_counter()
-    c1, h1 = cycles_to_periodic(first, period, first_tolerance, max_cycles,
-                                stop_on_plateau=True)
-    first.synchronize()
-    t1 = time.perf_counter()
-    final.set_distributions(first.get_distributions())
-    final.SetTimeStep(first.GetTimeStep())
-    final.synchronize()
-    t2 = time.perf_counter()
-    c2, h2 = cycles_to_periodic(final, period, tolerance, max_cycles)
-    final.synchronize()
-    t3 = time.perf_counter()
-    return {"cycles_first": c1, "changes_first": h1, "seconds_first": t1 - t0,
-            "handover_seconds": t2 - t1, "cycles": c2, "changes": h2,
-            "seconds": t3 - t0, "converged": c2 is not None}
+    """periodic_state_staged with two stages: the transient in `first`, the
+    periodic state in `final`.  Returns the cycles, changes and wall times
+    of both."""
+    r = periodic_state_staged([first, final], period, tolerance, first_tolerance, max_cycles)
+    a, b = r["stages"]
+    return {"cycles_first": a["cycles"], "changes_first": a["changes"],
+            "seconds_first": a["seconds"], "handover_seconds": r["handover_seconds"],
+            "cycles": b["cycles"], "changes": b["changes"], "seconds": r["seconds"],
+            "converged": r["converged"]}
 
 
 # ---------------------------------------------------------------------------
