@@ -169,19 +169,6 @@ def periodic_state_staged(stages: list, period: int, tolerance: float = 1e-4,
             "converged": out[-1]["cycles"] is not None}
 
 
-def periodic_state_continued(first: LBM, final: LBM, period: int, tolerance: float = 1e-4,
-                             first_tolerance: float = 1e-2, max_cycles: int = 12) -> dict:
-    """periodic_state_staged with two stages: the transient in `first`, the
-    periodic state in `final`.  Returns the cycles, changes and wall times
-    of both."""
-    r = periodic_state_staged([first, final], period, tolerance, first_tolerance, max_cycles)
-    a, b = r["stages"]
-    return {"cycles_first": a["cycles"], "changes_first": a["changes"],
-            "seconds_first": a["seconds"], "handover_seconds": r["handover_seconds"],
-            "cycles": b["cycles"], "changes": b["changes"], "seconds": r["seconds"],
-            "converged": r["converged"]}
-
-
 # ---------------------------------------------------------------------------
 # Extraction (.xtr) files and checkpoints
 # ---------------------------------------------------------------------------
