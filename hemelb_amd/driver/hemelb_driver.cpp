@@ -15,12 +15,15 @@
 // binary64 [n][Q] in the domain's site order.
 #include "hemelb_amd.hpp"
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 using namespace hemelb;
 
@@ -35,6 +38,11 @@ struct Options {
   double tau = 0.8, amplitude = 1e-3, period = 100.0;
   int64_t steps = 100, monitor_every = 0;
   std::string out;
+  // periodic-state mode: run to the periodic state instead of `steps`,
+  // optionally with the transient in faster storage forms (`stages`)
+  double to_periodic = 0.0, stage_tol = 1e-2;
+  int max_cycles = 12;
+  std::string stages;
 };
 
 [[noreturn]] void usage() {
@@ -42,7 +50,9 @@ struct Options {
                "usage: hemelb_driver [--cylinder R L | --gmy file] [--lattice 15|19]\n"
                "       [--kernel LBGK|MRT] [--wall SBB|BFL] [--storage fp64|fp32|fp16]\n"
                "       [--tau t] [--amplitude A] [--period T] [--steps n]\n"
-               "       [--monitor-every k] [--out file]\n");
+               "       [--monitor-every k] [--out file]\n"
+               "       [--to-periodic tol [--stages fp16,fp32,fp64] [--stage-tol t]\n"
+               "        [--max-cycles n]]\n");
   std::exit(2);
 }
 
@@ -79,6 +89,14 @@ Options parse(int argc, char** argv) {
       o.monitor_every = std::atoll(next().c_str());
     } else if (a == "--out") {
       o.out = next();
+    } else if (a == "--to-periodic") {
+      o.to_periodic = std::atof(next().c_str());
+    } else if (a == "--stages") {
+      o.stages = next();
+    } else if (a == "--stage-tol") {
+      o.stage_tol = std::atof(next().c_str());
+    } else if (a == "--max-cycles") {
+      o.max_cycles = std::atoi(next().c_str());
     } else {
       usage();
     }
@@ -95,6 +113,58 @@ lb::Storage storage_of(const std::string& s) {
 
 std::array<double, 3> a3(const double* v) { return {v[0], v[1], v[2]}; }
 
+void write_f(const std::string& path, const std::vector<distribn_t>& f) {
+  std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(path.c_str(), "wb"), &std::fclose);
+  if (!fp || std::fwrite(f.data(), sizeof(double), f.size(), fp.get()) != f.size())
+    throw Exception("cannot write " + path);
+}
+
+// From rest to the periodic state (cycle-to-cycle change below
+// o.to_periodic), the transient in the earlier storage forms of o.stages
+// and the state handed over exactly between them (PeriodicStateStaged).
+template <class T>
+int run_periodic(const Options& o, const lb::DomainArrays& dom,
+                 const std::vector<hlb_iolet>& inlets, const std::vector<hlb_iolet>& outlets) {
+  std::vector<std::string> forms;
+  const std::string list = o.stages.empty() ? o.storage : o.stages;
+  for (size_t b = 0; b <= list.size();) {
+    const size_t e = std::min(list.find(',', b), list.size());
+    forms.push_back(list.substr(b, e - b));
+    b = e + 1;
+  }
+  std::vector<std::unique_ptr<lb::LBM<T>>> owned;
+  std::vector<lb::LBM<T>*> stages;
+  for (const auto& form : forms) {
+    owned.push_back(std::make_unique<lb::LBM<T>>(lb::LbmParameters(o.tau), dom, inlets,
+                                                 outlets, storage_of(form)));
+    owned.back()->SetInitialConditions(1.0);
+    stages.push_back(owned.back().get());
+  }
+  const int64_t period = (int64_t)std::llround(o.period);
+  std::vector<int> cycles;
+  std::vector<std::vector<double>> changes;
+  const auto t0 = std::chrono::steady_clock::now();
+  const bool converged = lb::PeriodicStateStaged(stages, period, o.to_periodic, o.stage_tol,
+                                                 o.max_cycles, cycles, changes);
+  lb::LBM<T>& last = *stages.back();
+  const hlb_monitor m = last.Monitor();
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (!m.stable) throw Exception("simulation unstable");
+  if (!o.out.empty()) write_f(o.out, last.GetDistributions());
+  std::string sc;
+  for (size_t k = 0; k < forms.size(); ++k)
+    sc += (k ? ", " : "") + std::string("[\"") + forms[k] + "\", " + std::to_string(cycles[k]) + "]";
+  std::printf(
+      "{\"sites\": %lld, \"Q\": %d, \"time_step\": %llu, \"seconds\": %.6f, "
+      "\"converged\": %d, \"stage_cycles\": [%s], \"last_change\": %.6g, "
+      "\"mass\": %.17g, \"velocity_max\": %.17g, \"stable\": %d}\n",
+      (long long)last.TotalFluidSiteCount(), T::Lattice::NUMVECTORS,
+      (unsigned long long)last.GetTimeStep(), secs, converged ? 1 : 0, sc.c_str(),
+      changes.back().empty() ? 0.0 : changes.back().back(), m.mass, m.velocity_max, m.stable);
+  return converged ? 0 : 3;
+}
+
 template <class T>
 int run(const Options& o, const lb::DomainArrays& dom) {
   std::vector<hlb_iolet> inlets, outlets;
@@ -103,6 +173,7 @@ int run(const Options& o, const lb::DomainArrays& dom) {
                                         o.period));
   for (const auto& io : dom.outlets)
     outlets.push_back(lb::InOutLetCosine(a3(io.position), a3(io.normal), 1.0, 0.0, o.period));
+  if (o.to_periodic > 0.0) return run_periodic<T>(o, dom, inlets, outlets);
   lb::LBM<T> lbm(lb::LbmParameters(o.tau), dom, inlets, outlets, storage_of(o.storage));
   lbm.SetInitialConditions(1.0);
   const auto t0 = std::chrono::steady_clock::now();
@@ -124,12 +195,7 @@ int run(const Options& o, const lb::DomainArrays& dom) {
   const hlb_monitor m = lbm.Monitor();
   const double secs =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (!o.out.empty()) {
-    const std::vector<distribn_t> f = lbm.GetDistributions();
-    std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(o.out.c_str(), "wb"), &std::fclose);
-    if (!fp || std::fwrite(f.data(), sizeof(double), f.size(), fp.get()) != f.size())
-      throw Exception("cannot write " + o.out);
-  }
+  if (!o.out.empty()) write_f(o.out, lbm.GetDistributions());
   std::printf(
       "{\"sites\": %lld, \"Q\": %d, \"steps\": %lld, \"time_step\": %llu, \"seconds\": %.6f, "
       "\"mass\": %.17g, \"density_min\": %.17g, \"density_max\": %.17g, "

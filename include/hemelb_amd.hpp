@@ -379,6 +379,14 @@ class LBM {
     check(hlb_reduce(s_, &m), "hlb_reduce");
     return m;
   }
+  // Periodic-state monitor: SnapshotVelocity keeps the current velocity
+  // field on the device; VelocityChange is max |u - u_kept| / reference_speed.
+  void SnapshotVelocity() { check(hlb_snapshot_velocity(s_), "hlb_snapshot_velocity"); }
+  double VelocityChange(double reference_speed) {
+    double v = 0.0;
+    check(hlb_velocity_change(s_, reference_speed, &v), "hlb_velocity_change");
+    return v;
+  }
   hlb_solver* Handle() { return s_; }
 
  private:
@@ -386,6 +394,54 @@ class LBM {
   site_t n_;
   hlb_solver* s_ = nullptr;
 };
+
+// Inlet cycles of `period` steps run until the cycle-to-cycle change
+// max |u(t) - u(t - T)| / max |u| is below `tolerance` or, with
+// stop_on_plateau, a cycle no longer halves it (a reduced-precision form's
+// rounding floor).  Appends every cycle's change to `changes`; returns the
+// cycles run when it stopped for either reason, 0 after max_cycles.
+template <class T>
+int CyclesToPeriodic(LBM<T>& lbm, int64_t period, double tolerance, int max_cycles,
+                     bool stop_on_plateau, std::vector<double>& changes) {
+  lbm.SnapshotVelocity();
+  const size_t first = changes.size();
+  for (int cycles = 1; cycles <= max_cycles; ++cycles) {
+    lbm.Step(period);
+    const double umax = lbm.Monitor().velocity_max;
+    const double change = lbm.VelocityChange(umax > 1e-30 ? umax : 1e-30);
+    changes.push_back(change);
+    lbm.SnapshotVelocity();
+    const bool plateau = stop_on_plateau && changes.size() > first + 1 &&
+                         change > 0.5 * changes[changes.size() - 2];
+    if (change < tolerance || plateau) return cycles;
+  }
+  return 0;
+}
+
+// The periodic state of the last solver of `stages` with the transient run
+// in the earlier, faster (reduced-precision) ones, all of the same domain
+// and iolets.  Each earlier stage runs until its change is below
+// stage_tolerance or stops halving; its distributions (exact in binary64)
+// and time step then start the next.  The last stage runs until its own
+// change is below `tolerance`.  `cycles[k]` and `changes[k]` report stage
+// k; returns whether the last stage converged.
+template <class T>
+bool PeriodicStateStaged(const std::vector<LBM<T>*>& stages, int64_t period, double tolerance,
+                         double stage_tolerance, int max_cycles, std::vector<int>& cycles,
+                         std::vector<std::vector<double>>& changes) {
+  cycles.assign(stages.size(), 0);
+  changes.assign(stages.size(), {});
+  for (size_t k = 0; k < stages.size(); ++k) {
+    const bool last = k + 1 == stages.size();
+    if (k) {
+      stages[k]->SetDistributions(stages[k - 1]->GetDistributions());
+      stages[k]->SetTimeStep(stages[k - 1]->GetTimeStep());
+    }
+    cycles[k] = CyclesToPeriodic(*stages[k], period, last ? tolerance : stage_tolerance,
+                                 max_cycles, !last, changes[k]);
+  }
+  return cycles.back() > 0;
+}
 
 }  // namespace lb
 }  // namespace hemelb

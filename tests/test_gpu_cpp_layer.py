@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from hemelb_amd import extraction as X
 from hemelb_amd.geometry import Geometry
 from tests import cases as K
 
@@ -57,3 +58,28 @@ def test_driver_reduced_storage_matches_the_python_actor(tmp_path):
     lbm = case.product(storage="fp32")
     lbm.step(steps)
     np.testing.assert_array_equal(f.reshape(dom.n, 19), lbm.get_distributions())
+
+
+def test_driver_staged_periodic_state_matches_the_python_helper(tmp_path):
+    """The driver's periodic-state mode (PeriodicStateStaged of the template
+    layer: the transient in binary16, one binary32 stage, binary64 to the
+    1e-4 tolerance) makes the same library calls as
+    hemelb_amd.extraction.periodic_state_staged: the same cycles per stage,
+    the same time step and the same final state, bit for bit."""
+    R, length, tau, amp, period = 6.0, 24, 0.7, 0.004, 1000
+    chain = ["fp16", "fp32", "fp64"]
+    info, f = _driver(tmp_path, "--cylinder", R, length, "--lattice", 19, "--wall", "BFL",
+                      "--tau", tau, "--amplitude", amp, "--period", period,
+                      "--to-periodic", 1e-4, "--stages", ",".join(chain))
+    dom = Geometry.cylinder(R, length).project(19)
+    ins, outs = K.pressure_iolets(dom, 1.0, amp, float(period), 1.0)
+    case = K.Case(dom, "LBGK", "BFL", "NASH", "NASH", tau, ins, outs, rho0=1.0)
+    stages = [case.product(storage=st) for st in chain]
+    r = X.periodic_state_staged(stages, period, 1e-4, 1e-2, 12)
+    print(f"[cpp] staged: driver {info['stage_cycles']}, python "
+          f"{[x['cycles'] for x in r['stages']]}", flush=True)
+    assert info["converged"] == 1 and r["converged"] and info["stable"] == 1
+    assert [c for _, c in info["stage_cycles"]] == [x["cycles"] for x in r["stages"]]
+    assert info["time_step"] == stages[-1].GetTimeStep()
+    np.testing.assert_array_equal(f.reshape(dom.n, 19), stages[-1].get_distributions())
+
